@@ -1,0 +1,149 @@
+/* scopyon HIP C-ABI (libscopyon_hip.so, gfx950).
+ *
+ * The reference (ecell/scopyon, pure Python) has no FFI: its hot path sits
+ * behind Python functions.  Each entry point below replaces one reference
+ * loop or function; the citation names it (src/scopyon/<file>:<lines>).  The
+ * Python package `scopyon_amd` binds these through ctypes and keeps the
+ * reference's Python signatures on top.
+ *
+ * Conventions
+ *   - All pointers are DEVICE pointers unless the comment says "host".
+ *   - Every call is asynchronous on `stream`; nothing allocates device memory:
+ *     scratch space comes in through `workspace` arguments sized by the
+ *     matching *_workspace_bytes() query.
+ *   - Return value: 0 on success, negative on error (SMX_EVALUE -1,
+ *     SMX_ETYPE -2, SMX_ERUNTIME -3, SMX_ENUMERIC -4); smx_last_error()
+ *     returns a thread-local message.  Data-dependent failures found inside
+ *     kernels are reported through a device `status` word (bit flags) that the
+ *     caller reads after synchronising.
+ *   - float64 throughout the parity paths; the library is compiled with
+ *     -ffp-contract=off so float64 expressions round like numpy's.
+ */
+#ifndef SCOPYON_HIP_H
+#define SCOPYON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#include "scopyon_hip_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- plumbing ----------------------------------------------------------- */
+const char* smx_last_error(void);
+int smx_version(void);
+int smx_device_count(int* count /* host */);
+/* Host evaluations of the device helpers (for CPU-side tests). */
+void smx_philox4x32_10_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double smx_i1e_host(double y);                       /* scipy.special.i1e, _epifm.py:13,369 */
+double smx_np_remainder_host(double a, double b);    /* numpy '%', sampling2.py:46-47 */
+size_t smx_sat_index_host(int side, int period, int a, int b);
+
+/* ---- PSF tables (_epifm.py:76-213) -------------------------------------- */
+/* Born-Wolf radial profile for nz depths: out[z * nr + r], r = k * res.
+ * Replaces PointSpreadingFunction.__get_born_wolf_distribution_old
+ * (_epifm.py:180-213).  alpha = 2 pi NA / lambda, gamma = (2 pi / lambda)(NA/2)^2,
+ * scale = alpha^2 / pi, all computed by the caller in float64.
+ * j0_work: nr * 100 doubles. */
+int smx_psf_radial_born_wolf(const double* depths, int nz, int nr, double res, double alpha,
+                             double gamma, double scale, double* j0_work, double* out,
+                             hipStream_t stream);
+/* Gaussian radial profile (_epifm.py:128-134). */
+int smx_psf_radial_gaussian(int nr, double res, double width, double* out, hipStream_t stream);
+/* Summed-area tables of the (2 nr - 1)^2 cartesian tables
+ * (radial_to_cartesian, _epifm.py:98-126) for nz profiles, stored with
+ * smx_psf_sat_elems(nr, period) doubles per table. */
+size_t smx_psf_sat_elems(int nr, int period);
+size_t smx_psf_sat_workspace_bytes(int nr, int nz);
+int smx_psf_sat_build(const double* radial, int nz, int nr, double res, int period,
+                      double* workspace, double* sat, hipStream_t stream);
+
+/* ---- emitters and splat (_epifm.py:1267-1333, 215-282) ------------------ */
+/* Per-emitter weight w (the reference's `normalization`) and depth-table key
+ * for one sub-step of length unit_time.  Replaces the scalar part of
+ * _EPIFMSimulator.__overlay_molecule_plane (_epifm.py:1280-1315); with
+ * `budget` non-NULL also the photobleaching budget update (_epifm.py:1294-1306).
+ * depth = particle column 0 (absolute); p_state may be NULL (all ones);
+ * slot_map (may be NULL) maps depth-key index (0.., cutoff_slot) to table slot. */
+int smx_emitter_weights(int n, const double* depth, const double* p_state,
+                        const smx_weight_params_t* params /* host */, double unit_time,
+                        double* budget, const int* budget_slot, const int* slot_map,
+                        int n_slot_map, double* w_out, int* key_out, double* pstate_out,
+                        hipStream_t stream);
+/* Tiles of a frame (16 x 16 pixels) and the splat workspace size. */
+int smx_splat_tiles(const smx_fov_t* fov /* host */, int reserved);
+int smx_splat_tile_size(void);
+size_t smx_splat_workspace_bytes(int n_tiles, long long list_capacity);
+/* Pixel integration of n emitters into nfov frames: replaces
+ * PointSpreadingFunction.overlay_signal_ called per emitter
+ * (_epifm.py:223-282 via _epifm.py:1261-1264).  u, v: absolute lateral
+ * coordinates; w: weights; key: SAT slot per emitter; fov_id: FOV per emitter
+ * (NULL: all in FOV 0).  sat: tables of sat_stride doubles each.
+ * accumulate=1 adds into `out` (camera += expected); 0 overwrites.
+ * status bit 1: a tile list exceeded the LDS sort capacity (still correct,
+ * order not by emitter index); bit 2: list capacity too small (result
+ * incomplete); bit 8: an emitter's table slot was not built (skipped). */
+int smx_splat_expected(int n, const double* u, const double* v, const double* w, const int* key,
+                       const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
+                       const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
+                       double res, double unit_area, long long list_capacity, int accumulate,
+                       double* out, void* workspace, size_t workspace_bytes, int* status_dev,
+                       hipStream_t stream);
+
+/* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */
+/* E = QE * (photons + background) (_epifm.py:1436-1441); flag_out (may be
+ * NULL) = E > 0, the pixels that consume a uniform in the EMCCD draw. */
+int smx_detector_expectation(int n, const double* photons,
+                             const smx_detector_params_t* params /* host */, double* E_out,
+                             int* flag_out, hipStream_t stream);
+size_t smx_scan_workspace_bytes(int n);
+int smx_exclusive_scan_i32(int n, const int* in, int* out, void* workspace, size_t bytes,
+                           hipStream_t stream);
+/* Total pmf mass for one expectation value (written to *T_out, device). */
+int smx_emccd_common_total(double E, int emgain, double* T_out, hipStream_t stream);
+/* EMCCD.get_signal + ADC with caller-drawn random numbers
+ * (_epifm.py:393-416 and 1472-1484): noise[i] = readout noise (NULL: 0),
+ * uniforms[uidx[i]] = the uniform RandomState.choice draws for pixel i.
+ * Pixels with E == E_common reuse *T_common (may be NULL).  status bit 4:
+ * the pmf overflowed for some pixel (the reference raises ValueError). */
+int smx_emccd_parity(int n, const double* E, const double* noise, const double* uniforms,
+                     const int* uidx, const double* gain_map, const double* offset_map,
+                     const smx_detector_params_t* params /* host */, double E_common,
+                     const double* T_common, double* adc_out, double* signal_out,
+                     int* status_dev, hipStream_t stream);
+/* EMCCD with Philox4x32-10 randomness: Poisson(E) -> Gamma(n, emgain) signal,
+ * Gaussian readout noise, ADC.  Fuses _epifm.py:1436-1469. */
+int smx_emccd_philox(int n, const double* photons, const double* gain_map,
+                     const double* offset_map, const smx_detector_params_t* params /* host */,
+                     uint64_t seed, uint32_t frame, uint32_t pix_base, double* E_out,
+                     void* adc_out, int adc_is_float32, hipStream_t stream);
+
+/* ---- Brownian sampling (sampling2.py:33-68, sampling.py:88-119) --------- */
+/* Whole trajectories from caller-drawn normals xi[t][i][d] (and transition
+ * uniforms trans_u[t][i]): x += scale[t][state][d] * xi; boundary 0 none,
+ * 1 periodic wrap with numpy '%' semantics, 2 reflecting walls (device
+ * lower/upper); optional state transitions
+ * searchsorted(pacc[t][state], u, 'left').  xs[(t+1)][i][d], states[t][i]. */
+int smx_brownian_predrawn(int n, int ndim, int nsteps, int nstates, const double* x0,
+                          const int* state0, const double* scale, const double* xi, int boundary,
+                          const double* lower, const double* upper, const double* pacc,
+                          const double* trans_u, double* xs, int* states, hipStream_t stream);
+/* One Philox step on structure-of-arrays coordinates: x += sqrt(2 D dt) N(0,1);
+ * boundary 0 none, 1 periodic, 2 reflecting.  lower/upper: host, ndim each. */
+int smx_brownian_philox(int n, int ndim, double* x0, double* x1, double* x2, const double* D,
+                        double D_scalar, double dt, int boundary, const double* lower,
+                        const double* upper, uint64_t seed, uint32_t step, uint32_t id_base,
+                        hipStream_t stream);
+/* Uniform initial positions (sampling2.py:15-31) from Philox; lower/upper host. */
+int smx_uniform_init(int n, int ndim, double* x0, double* x1, double* x2, const double* lower,
+                     const double* upper, uint64_t seed, uint32_t seq, uint32_t id_base,
+                     hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCOPYON_HIP_H */

@@ -1,0 +1,61 @@
+/* Plain-old-data structs shared by the scopyon HIP C-ABI and its kernels.
+ * All lengths are SI metres, times seconds; all doubles are float64. */
+#ifndef SCOPYON_HIP_TYPES_H
+#define SCOPYON_HIP_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One field of view (camera frame) in a batch.
+ * Image layout: pixel (i, j) of FOV f is out[pix_off + i * nh + j], i.e. the
+ * reference's camera[i, j] with i along image_size[0] (u) and j along
+ * image_size[1] (v) (src/scopyon/_epifm.py:1175-1177). */
+typedef struct smx_fov_t {
+    int32_t nw, nh;          /* image_size = (nw, nh) */
+    int32_t tiles_r, tiles_c;/* ceil(nw / 16), ceil(nh / 16) */
+    int64_t pix_off;         /* first pixel of this frame in the batch buffer */
+    int32_t tile_off;        /* first global tile id of this frame */
+    int32_t reserved;
+    double pixel_length;     /* detector pixel / magnification (_epifm.py:1176) */
+    double half_w, half_h;   /* (nw * pixel_length) * 0.5, (nh * pixel_length) * 0.5 */
+    double p0u, p0v;         /* focal point lateral coordinates (_epifm.py:1168) */
+} smx_fov_t;
+
+/* Per-emitter weight constants, all computed on the host in float64 with the
+ * reference's operation order (src/scopyon/_epifm.py:1343-1428). */
+typedef struct smx_weight_params_t {
+    double amplitude;              /* snells_law() amplitude [photons/s/m^2] */
+    double penetration_depth;      /* snells_law() depth [m] (inf for EPI) */
+    double x_sec;                  /* ln(10) * abs_coeff * 0.1 / N_A */
+    double quantum_yield;
+    double one_minus_10_pow_minus_a; /* 1 - 10**(-A) */
+    double spectral_psf_norm;      /* numpy.sum(fluoem_norm) * psf_normalization */
+    double four_pi;                /* 4.0 * numpy.pi */
+    double focal_depth;            /* p_0[0] */
+    double depth_cutoff;           /* fluorophore.depth_cutoff */
+    double res;                    /* 1e-9, depth key resolution */
+    int32_t cutoff_slot;           /* table slot used for |depth| >= cutoff + res */
+    int32_t reserved;
+} smx_weight_params_t;
+
+/* Detector constants (src/scopyon/_epifm.py:1430-1484). */
+typedef struct smx_detector_params_t {
+    double qe;                 /* detector.QE */
+    double background;         /* effects.background.mean, 0 if switched off */
+    int32_t background_switch;
+    int32_t emgain;            /* detector.emgain (EMCCD) */
+    double readout_noise;      /* detector.readout_noise (e-) */
+    double fullwell;           /* ADC fullwell */
+    double adc_max;            /* 2**bit - 1 */
+    double offset;             /* scalar ADC offset (used when offset_map is NULL) */
+    double gain;               /* scalar ADC gain (used when gain_map is NULL) */
+} smx_detector_params_t;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SCOPYON_HIP_TYPES_H */

@@ -1,0 +1,126 @@
+"""Device-resident PSF tables: radial profiles and summed-area tables (SATs),
+one per 1 nm depth key, built on first use like the reference's per-key cache
+(src/scopyon/_epifm.py:76-96) but kept in HBM across frames and calls.
+
+A SAT holds the (2 n - 1)^2 cartesian table of ``radial_to_cartesian``
+(_epifm.py:98-126) as corner prefix sums; the splat kernel integrates any
+pixel rectangle from four of its entries.  One table is 32 MB at the default
+1 nm grid, so even all 1002 depth keys (32 GB) fit in one MI355X's HBM.
+"""
+import math
+
+import numpy
+import torch
+
+from . import _device, _lib
+
+RES = 1.0e-9   # _epifm.py:59-60
+
+
+def depth_keys(depth, focal_depth, depth_cutoff):
+    """Vectorised ``PointSpreadingFunction.get`` key (_epifm.py:76-85); -1 = cutoff."""
+    ad = numpy.abs(numpy.asarray(depth, dtype=numpy.float64) - focal_depth)
+    keys = numpy.full(ad.shape, -1, dtype=numpy.int64)
+    inside = ad < depth_cutoff + RES
+    keys[inside] = (ad[inside] / RES).astype(numpy.int64)
+    return keys
+
+
+def sat_period(pixel_length):
+    """Layout period: the pixel length in table cells when it is an integer."""
+    ratio = pixel_length / RES
+    p = int(round(ratio))
+    if 1 <= p <= 4096 and abs(ratio - p) < 1e-6:
+        return p
+    return 1
+
+
+class PSFTables:
+    """Per-depth-key SATs for one fluorophore/PSF model on one device."""
+
+    def __init__(self, fluorophore_type, wavelength, radial_cutoff, depth_cutoff,
+                 radial_width=None, pixel_length=None, device=None, na=1.4):
+        self.device = device or _device.require_gpu()
+        self.gaussian = fluorophore_type == 'Gaussian'
+        if self.gaussian and radial_width is None:
+            raise ValueError('fluorophore.radial_width must be given for Gaussian type fluorophore.')
+        self.wavelength = wavelength
+        self.radial_width = radial_width
+        self.depth_cutoff = depth_cutoff
+        self.nr = len(numpy.arange(0.0, radial_cutoff, RES, dtype=float))
+        self.ntab = 2 * self.nr - 1
+        self.period = sat_period(pixel_length) if pixel_length else 1
+        self.sat_elems = int(_lib.lib().smx_psf_sat_elems(self.nr, self.period))
+        # NA is 1.4 in the reference (_epifm.py:183); alpha, gamma as at :186-188
+        k = 2.0 * numpy.pi / wavelength
+        self.alpha = k * na
+        self.gamma = k * numpy.power(na / 2, 2)
+        self.scale = self.alpha * self.alpha / numpy.pi
+        # key (0..n_keys-2 for depth keys, n_keys-1 for the cutoff key) -> slot
+        self.n_keys = int(depth_cutoff / RES) + 2
+        self.cutoff_index = self.n_keys - 1
+        self.slot_of = {}
+        self.capacity = 0
+        self.sat = None
+
+    def key_index(self, key):
+        return self.cutoff_index if key < 0 else int(key)
+
+    def _depth_of(self, key):
+        return self.depth_cutoff if key < 0 else key * RES
+
+    def _grow(self, need):
+        if need <= self.capacity:
+            return
+        cap = max(need, 2 * self.capacity, 1)
+        new = torch.empty((cap, self.sat_elems), dtype=torch.float64, device=self.device)
+        if self.sat is not None and self.capacity:
+            new[: self.capacity].copy_(self.sat[: self.capacity])
+        self.sat, self.capacity = new, cap
+
+    def ensure(self, keys):
+        """Build the tables for every key in ``keys`` not built yet."""
+        if self.gaussian:
+            keys = [0]   # the Gaussian PSF does not depend on depth (_epifm.py:133)
+        missing = sorted({int(k) for k in numpy.unique(keys)} - set(self.slot_of))
+        if not missing:
+            return
+        base = len(self.slot_of)
+        self._grow(base + len(missing))
+        lib, st = _lib.lib(), _device.stream()
+        nz = len(missing)
+        radial = torch.empty((nz, self.nr), dtype=torch.float64, device=self.device)
+        if self.gaussian:
+            _lib.call('smx_psf_radial_gaussian', self.nr, RES, self.radial_width,
+                      radial.data_ptr(), st)
+        else:
+            depths = _device.upload(numpy.array([self._depth_of(k) for k in missing]),
+                                    device=self.device)
+            j0w = _device.ARENA.typed('psf_j0', self.nr * 100, torch.float64, self.device)
+            _lib.call('smx_psf_radial_born_wolf', depths.data_ptr(), nz, self.nr, RES, self.alpha,
+                      self.gamma, self.scale, j0w.data_ptr(), radial.data_ptr(), st)
+        # build in batches so the row-sum scratch stays bounded (~64 MB per table)
+        batch = 8
+        for s in range(0, nz, batch):
+            m = min(batch, nz - s)
+            ws = _device.ARENA.get('psf_satws', lib.smx_psf_sat_workspace_bytes(self.nr, m),
+                                   self.device)
+            _lib.call('smx_psf_sat_build', radial[s:s + m].data_ptr(), m, self.nr, RES,
+                      self.period, ws.data_ptr(), self.sat[base + s].data_ptr(), st)
+        for i, k in enumerate(missing):
+            self.slot_of[k] = base + i
+        self.radial_last = radial
+
+    def slot_map(self):
+        """Device int32 table: key index -> slot (-1 if not built)."""
+        m = numpy.full(self.n_keys, -1, dtype=numpy.int32)
+        for k, s in self.slot_of.items():
+            m[self.key_index(k)] = s
+        if self.gaussian:
+            m[:] = self.slot_of.get(0, -1)
+        return _device.upload(m, dtype=torch.int32, device=self.device)
+
+    def prebuild_all(self):
+        """Build every depth key (0 .. cutoff) plus the cutoff key."""
+        keys = list(range(self.n_keys - 1)) + [-1]
+        self.ensure(keys)

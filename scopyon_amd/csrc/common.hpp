@@ -1,0 +1,152 @@
+// Shared device helpers for the scopyon HIP kernels (gfx950, wave64).
+//
+// Compiled with -ffp-contract=off: every a*b+c below rounds twice, exactly
+// like the float64 numpy expressions it mirrors.  Where a fused multiply-add
+// is wanted it is written as fma().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "i1e_coeffs.hpp"
+
+#define SMX_HD __host__ __device__ inline
+
+// ---------------------------------------------------------------------------
+// Status codes shared with include/scopyon_hip.h
+// ---------------------------------------------------------------------------
+#define SMX_OK 0
+#define SMX_EVALUE (-1)
+#define SMX_ETYPE (-2)
+#define SMX_ERUNTIME (-3)
+#define SMX_ENUMERIC (-4)
+
+namespace smx {
+
+void set_error(const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+
+#define SMX_CHECK_LAUNCH(what)                                          \
+    do {                                                                \
+        int _rc = ::smx::check_hip(hipGetLastError(), what);            \
+        if (_rc) return _rc;                                            \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:
+// as easy as 1, 2, 3").  Counter-based: (counter, key) -> 4 x 32 random bits.
+// ---------------------------------------------------------------------------
+struct u32x4 { uint32_t x, y, z, w; };
+
+SMX_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * c.x;
+        uint64_t p1 = (uint64_t)M1 * c.z;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+SMX_HD double u01_53(uint32_t a, uint32_t b) {
+    uint64_t v = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);
+    return (double)(v & ((1ull << 53) - 1)) * 0x1.0p-53;
+}
+// 53-bit uniform in (0, 1].
+SMX_HD double u01_53_open0(uint32_t a, uint32_t b) { return 1.0 - u01_53(a, b); }
+
+// A small per-thread stream: counter (id, seq, ctr2, draw index), fixed key.
+struct PhiloxStream {
+    uint32_t k0, k1, id, seq, lane2, n;
+    u32x4 buf;
+    int left;
+    SMX_HD PhiloxStream(uint64_t seed, uint32_t id_, uint32_t seq_, uint32_t lane2_)
+        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), id(id_), seq(seq_), lane2(lane2_),
+          n(0), buf{0, 0, 0, 0}, left(0) {}
+    SMX_HD void refill() {
+        buf = philox4x32_10(u32x4{id, seq, lane2, n}, k0, k1);
+        ++n;
+        left = 4;
+    }
+    SMX_HD uint32_t next32() {
+        if (left == 0) refill();
+        --left;
+        return left == 3 ? buf.x : left == 2 ? buf.y : left == 1 ? buf.z : buf.w;
+    }
+    SMX_HD double uniform() { uint32_t a = next32(); uint32_t b = next32(); return u01_53(a, b); }
+    SMX_HD double uniform_open0() { return 1.0 - uniform(); }
+};
+
+// ---------------------------------------------------------------------------
+// i1e(y) = exp(-y) I1(y), y >= 0 (coefficients from gen_i1e.py).
+// ---------------------------------------------------------------------------
+SMX_HD double i1e(double y) {
+    if (y <= SMX_I1E_YB) {
+        double q = y * y * 0.25;
+        double acc = smx_i1e_series[SMX_I1E_NSERIES - 1];
+        for (int k = SMX_I1E_NSERIES - 2; k >= 0; --k) acc = acc * q + smx_i1e_series[k];
+        return (y * 0.5) * acc * exp(-y);
+    }
+    double t = (2.0 * SMX_I1E_YB) / y - 1.0;
+    double t2 = 2.0 * t, b1 = 0.0, b2 = 0.0;
+    for (int k = SMX_I1E_NCHEB - 1; k >= 1; --k) {
+        double b0 = t2 * b1 - b2 + smx_i1e_cheb[k];
+        b2 = b1;
+        b1 = b0;
+    }
+    double h = t * b1 - b2 + smx_i1e_cheb[0];
+    return h / sqrt(y);
+}
+
+// ---------------------------------------------------------------------------
+// numpy float semantics used by the parity paths
+// ---------------------------------------------------------------------------
+// numpy.remainder(a, b) for float64 (floor-division remainder, sign of b).
+SMX_HD double np_remainder(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0.0) != (m < 0.0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+// Ceil of a double, returned as a 64-bit integer (numpy.ceil(..).astype(int)).
+SMX_HD long long ceil_i64(double x) { return (long long)ceil(x); }
+
+// Summed-area-table index for a (side x side) corner grid stored either
+// row-major (period == 1) or phase-major with the given period: corners whose
+// row/column indices share residues mod `period` are contiguous, so one
+// emitter's regular pixel-boundary grid reads a compact block.
+struct SatLayout {
+    int side;     // corner indices 0..side-1 on each axis (table side + 1)
+    int period;   // >= 1
+    int g;        // ceil(side / period)
+    SMX_HD size_t index(int a, int b) const {
+        if (period == 1) return (size_t)a * side + b;
+        int pa = a % period, pb = b % period;
+        return ((size_t)(pa * period + pb) * g + (a / period)) * g + (b / period);
+    }
+    SMX_HD size_t elems() const {
+        return period == 1 ? (size_t)side * side : (size_t)period * period * g * g;
+    }
+};
+
+SMX_HD SatLayout make_sat_layout(int side, int period) {
+    SatLayout L;
+    L.side = side;
+    L.period = period < 1 ? 1 : period;
+    L.g = (side + L.period - 1) / L.period;
+    return L;
+}
+
+}  // namespace smx

@@ -1,0 +1,336 @@
+// Detector model: photons -> expectation E -> EMCCD signal + readout noise ->
+// ADC counts.
+//
+// Reference: src/scopyon/_epifm.py:1430-1470 (__detector_output),
+//            _epifm.py:355-416 (EMCCD), _epifm.py:1472-1484 (ADC).
+//
+// Two signal paths:
+//  * parity  : the reference's discrete pmf on S in [s_min, s_max) drawn by
+//              inverse CDF with a given uniform per pixel (the uniform numpy's
+//              RandomState.choice would consume).  One wave per pixel: pass 1
+//              sums the pmf, pass 2 scans it until the cumulative mass passes
+//              u * total.  Readout noise comes in pre-drawn.
+//  * philox  : statistically equivalent sampler driven by Philox4x32-10:
+//              n ~ Poisson(E), S = round(Gamma(n, emgain)) (S >= 1 when n >= 1),
+//              noise ~ N(0, readout_noise^2).  O(1) per pixel.
+#include "common.hpp"
+#include "scopyon_hip_types.h"
+
+#include <hipcub/hipcub.hpp>
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586;
+
+__device__ inline double adc_counts(double x, const smx_detector_params_t& P, double gain,
+                                    double offset) {
+    // _epifm.py:1475-1484, same order, no rounding
+    if (x > P.fullwell) x = P.fullwell;
+    x = x / gain;
+    x = x + offset;
+    if (x > P.adc_max) x = P.adc_max;
+    if (x < 0.0) x = 0.0;
+    return x;
+}
+
+__device__ inline double expectation(double photons, const smx_detector_params_t& P) {
+    if (P.background_switch) photons = photons + P.background;
+    return P.qe * photons;
+}
+
+// Scaled compound Poisson-Gamma pmf term (_epifm.py:364-388).
+__device__ inline double emccd_term(long long S, double E, double a) {
+    if (S == 0) return 1.0 / (2.0 * a * E);
+    double X = a * (double)S;
+    double Y = 2.0 * sqrt(E * X);
+    return 1.0 / Y * exp(-X + Y) * smx::i1e(Y);
+}
+
+__device__ inline void emccd_support(double E, int g, long long& s_min, long long& s_max) {
+    double sigma = sqrt(E) * 5.0 + 10.0;
+    long long lo = (long long)g * (long long)(E - sigma);   // int() truncates toward 0
+    s_min = lo > 0 ? lo : 0;
+    s_max = (long long)g * (long long)(E + sigma);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Pass 1 of the parity draw: total mass of the pmf for E (wave-cooperative).
+__device__ inline double emccd_total(double E, int g, double a, int lane) {
+    long long s_min, s_max;
+    emccd_support(E, g, s_min, s_max);
+    double acc = 0.0;
+    for (long long S = s_min + lane; S < s_max; S += 64) acc += emccd_term(S, E, a);
+    return wave_sum(acc);
+}
+
+__global__ void prepare_kernel(int n, const double* __restrict__ photons, smx_detector_params_t P,
+                               double* __restrict__ E_out, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double E = expectation(photons[i], P);
+    E_out[i] = E;
+    if (flag) flag[i] = E > 0.0 ? 1 : 0;
+}
+
+// T_common = total pmf mass for one expectation value shared by many pixels.
+__global__ void emccd_common_total_kernel(double E, int g, double* __restrict__ T_out) {
+    int lane = threadIdx.x & 63;
+    double a = 1.0 / (double)g;
+    double T = E > 0.0 ? emccd_total(E, g, a, lane) : 0.0;
+    if (lane == 0) *T_out = T;
+}
+
+__global__ void __launch_bounds__(256) emccd_parity_kernel(
+        int n, const double* __restrict__ E_arr, const double* __restrict__ noise,
+        const double* __restrict__ uniforms, const int* __restrict__ uidx,
+        const double* __restrict__ gain_map, const double* __restrict__ offset_map,
+        smx_detector_params_t P, double E_common, const double* __restrict__ T_common,
+        double* __restrict__ adc_out, double* __restrict__ signal_out, int* __restrict__ status) {
+    const int lane = threadIdx.x & 63;
+    const int pix = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pix >= n) return;
+    const int g = P.emgain;
+    const double a = 1.0 / (double)g;
+    const double E = E_arr[pix];
+    double S = 0.0;
+    if (!(E <= 1.0e4)) {
+        // NaN (int(nan) raises in the reference) or far beyond E ~ 709.8, where
+        // exp(-X + Y) overflows and numpy.random.choice raises ValueError
+        if (lane == 0) atomicOr(status, 4);
+    } else if (E > 0.0) {
+        long long s_min, s_max;
+        emccd_support(E, g, s_min, s_max);
+        const long long len = s_max - s_min;
+        double T = (T_common && E == E_common) ? *T_common : emccd_total(E, g, a, lane);
+        if (!(T > 0.0) || !(T < INFINITY)) {
+            // the reference's pmf overflows here (numpy.random.choice raises ValueError)
+            if (lane == 0) atomicOr(status, 4);
+        } else {
+            const double thr = uniforms[uidx[pix]] * T;
+            double carry = 0.0;
+            long long idx = len - 1;
+            for (long long c0 = 0; c0 < len; c0 += 64) {
+                long long k = c0 + lane;
+                double v = k < len ? emccd_term(s_min + k, E, a) : 0.0;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    double o = __shfl_up(v, off, 64);
+                    if (lane >= off) v += o;
+                }
+                double cum = carry + v;
+                unsigned long long hit = __ballot(k < len && cum > thr);
+                if (hit) {
+                    long long found = c0 + (long long)(__ffsll((long long)hit) - 1);
+                    idx = found < len - 1 ? found : len - 1;
+                    break;
+                }
+                carry = __shfl(cum, 63, 64);
+            }
+            S = (double)(s_min + idx);
+        }
+    }
+    if (lane == 0) {
+        double x = S + (noise ? noise[pix] : 0.0);
+        double gain = gain_map ? gain_map[pix] : P.gain;
+        double offset = offset_map ? offset_map[pix] : P.offset;
+        adc_out[pix] = adc_counts(x, P, gain, offset);
+        if (signal_out) signal_out[pix] = S;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Philox samplers
+// ---------------------------------------------------------------------------
+__device__ inline double normal01(smx::PhiloxStream& st) {
+    double u1 = st.uniform_open0();
+    double u2 = st.uniform();
+    double s, c;
+    sincos(kTwoPi * u2, &s, &c);
+    return sqrt(-2.0 * log(u1)) * c;
+}
+
+// Poisson: inversion below 12, Hormann's PTRS (transformed rejection with
+// squeeze, 1993) above.
+__device__ __noinline__ long long poisson_ptrs(smx::PhiloxStream& st, double lam);
+
+__device__ inline long long poisson(smx::PhiloxStream& st, double lam) {
+    if (!(lam > 0.0)) return 0;
+    if (lam >= 12.0) return poisson_ptrs(st, lam);
+    {
+        double u = st.uniform();
+        double p = exp(-lam), s = p;
+        long long k = 0;
+        while (u > s && k < 4096) {
+            ++k;
+            p *= lam / (double)k;
+            s += p;
+        }
+        return k;
+    }
+}
+
+__device__ __noinline__ long long poisson_ptrs(smx::PhiloxStream& st, double lam) {
+    const double slam = sqrt(lam), loglam = log(lam);
+    const double b = 0.931 + 2.53 * slam;
+    const double aa = -0.059 + 0.02483 * b;
+    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+    for (int it = 0; it < 1000; ++it) {
+        double U = st.uniform() - 0.5;
+        double V = st.uniform();
+        double us = 0.5 - fabs(U);
+        long long k = (long long)floor((2.0 * aa / us + b) * U + lam + 0.43);
+        if (us >= 0.07 && V <= vr) return k;
+        if (k < 0 || (us < 0.013 && V > us)) continue;
+        if (log(V) + log(invalpha) - log(aa / (us * us) + b) <=
+            -lam + (double)k * loglam - lgamma((double)k + 1.0))
+            return k;
+    }
+    return (long long)floor(lam + 0.5);
+}
+
+// Gamma(shape n >= 1, scale 1): exponential for n == 1, Marsaglia-Tsang
+// (ACM TOMS 26(3), 2000) otherwise.
+__device__ __noinline__ double gamma_int(smx::PhiloxStream& st, long long n) {
+    if (n == 1) return -log(st.uniform_open0());
+    const double d = (double)n - 1.0 / 3.0;
+    const double c = 1.0 / sqrt(9.0 * d);
+    for (int it = 0; it < 1000; ++it) {
+        double x = normal01(st);
+        double v = 1.0 + c * x;
+        if (v <= 0.0) continue;
+        v = v * v * v;
+        double u = st.uniform_open0();
+        if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v;
+        if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v;
+    }
+    return d;
+}
+
+__device__ inline double emccd_philox_signal(smx::PhiloxStream& st, double E, int g) {
+    if (!(E > 0.0)) return 0.0;
+    long long n = poisson(st, E);
+    if (n == 0) return 0.0;
+    double s;
+    int guard = 0;
+    do {
+        s = gamma_int(st, n) * (double)g;
+    } while (s < 0.5 && ++guard < 64);
+    double S = floor(s + 0.5);
+    return S < 1.0 ? 1.0 : S;
+}
+
+template <typename OutT>
+__global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
+                                    const double* __restrict__ gain_map,
+                                    const double* __restrict__ offset_map, smx_detector_params_t P,
+                                    uint64_t seed, uint32_t frame, uint32_t pix_base,
+                                    double* __restrict__ E_out, OutT* __restrict__ adc_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double E = expectation(photons[i], P);
+    smx::PhiloxStream st(seed, pix_base + (uint32_t)i, frame, 0x0E3CCD01u);
+    double noise = P.readout_noise > 0.0 ? P.readout_noise * normal01(st) : 0.0;
+    double S = emccd_philox_signal(st, E, P.emgain);
+    double gain = gain_map ? gain_map[i] : P.gain;
+    double offset = offset_map ? offset_map[i] : P.offset;
+    if (E_out) E_out[i] = E;
+    adc_out[i] = (OutT)adc_counts(S + noise, P, gain, offset);
+}
+
+}  // namespace
+
+extern "C" {
+
+int smx_detector_expectation(int n, const double* photons, const smx_detector_params_t* params,
+                             double* E_out, int* flag_out, hipStream_t stream) {
+    if (n < 0 || !params || (n > 0 && (!photons || !E_out))) {
+        smx::set_error("smx_detector_expectation: invalid arguments (n=%d)", n);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    hipLaunchKernelGGL(prepare_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, photons,
+                       *params, E_out, flag_out);
+    SMX_CHECK_LAUNCH("prepare_kernel");
+    return SMX_OK;
+}
+
+size_t smx_scan_workspace_bytes(int n) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int*)nullptr, (int*)nullptr, n);
+    return bytes;
+}
+
+int smx_exclusive_scan_i32(int n, const int* in, int* out, void* workspace, size_t bytes,
+                           hipStream_t stream) {
+    if (n < 0 || (n > 0 && (!in || !out || !workspace))) {
+        smx::set_error("smx_exclusive_scan_i32: invalid arguments (n=%d)", n);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    size_t need = smx_scan_workspace_bytes(n);
+    if (bytes < need) {
+        smx::set_error("smx_exclusive_scan_i32: workspace too small (%zu < %zu)", bytes, need);
+        return SMX_EVALUE;
+    }
+    if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(workspace, need, in, out, n, stream),
+                       "hipcub ExclusiveSum"))
+        return SMX_ERUNTIME;
+    return SMX_OK;
+}
+
+int smx_emccd_common_total(double E, int emgain, double* T_out, hipStream_t stream) {
+    if (emgain <= 0 || !T_out) {
+        smx::set_error("smx_emccd_common_total: invalid arguments (emgain=%d)", emgain);
+        return SMX_EVALUE;
+    }
+    hipLaunchKernelGGL(emccd_common_total_kernel, dim3(1), dim3(64), 0, stream, E, emgain, T_out);
+    SMX_CHECK_LAUNCH("emccd_common_total_kernel");
+    return SMX_OK;
+}
+
+int smx_emccd_parity(int n, const double* E, const double* noise, const double* uniforms,
+                     const int* uidx, const double* gain_map, const double* offset_map,
+                     const smx_detector_params_t* params, double E_common,
+                     const double* T_common, double* adc_out, double* signal_out,
+                     int* status_dev, hipStream_t stream) {
+    if (n < 0 || !params || params->emgain <= 0 ||
+        (n > 0 && (!E || !uniforms || !uidx || !adc_out || !status_dev))) {
+        smx::set_error("smx_emccd_parity: invalid arguments (n=%d)", n);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    hipLaunchKernelGGL(emccd_parity_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, n, E, noise,
+                       uniforms, uidx, gain_map, offset_map, *params, E_common, T_common, adc_out,
+                       signal_out, status_dev);
+    SMX_CHECK_LAUNCH("emccd_parity_kernel");
+    return SMX_OK;
+}
+
+int smx_emccd_philox(int n, const double* photons, const double* gain_map,
+                     const double* offset_map, const smx_detector_params_t* params,
+                     uint64_t seed, uint32_t frame, uint32_t pix_base, double* E_out,
+                     void* adc_out, int adc_is_float32, hipStream_t stream) {
+    if (n < 0 || !params || params->emgain <= 0 || (n > 0 && (!photons || !adc_out))) {
+        smx::set_error("smx_emccd_philox: invalid arguments (n=%d)", n);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    if (adc_is_float32)
+        hipLaunchKernelGGL(emccd_philox_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, stream,
+                           n, photons, gain_map, offset_map, *params, seed, frame, pix_base, E_out,
+                           (float*)adc_out);
+    else
+        hipLaunchKernelGGL(emccd_philox_kernel<double>, dim3((n + 255) / 256), dim3(256), 0,
+                           stream, n, photons, gain_map, offset_map, *params, seed, frame,
+                           pix_base, E_out, (double*)adc_out);
+    SMX_CHECK_LAUNCH("emccd_philox_kernel");
+    return SMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+"""GPU EMCCD + ADC vs the oracle with the same expectation and random numbers."""
+import numpy
+import pytest
+import torch
+
+from oracle import epifm as O
+from conftest import oracle_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _det_params(p):
+    from scopyon_amd import _lib
+    d = _lib.DetectorParamsT()
+    d.qe, d.background, d.background_switch = p.QE, p.background_mean, 1
+    d.emgain, d.readout_noise = p.emgain, p.readout_noise
+    d.fullwell, d.adc_max = p.adc_fullwell, 2 ** p.adc_bit - 1
+    d.offset, d.gain = float(p.adc_offset.flat[0]), float(p.adc_gain.flat[0])
+    return d
+
+
+def _gpu_parity(gpu, E, noise, uniforms, det, gain_map=None, offset_map=None):
+    """Run the parity EMCCD kernel on a given expectation array."""
+    from scopyon_amd import _lib, _device
+    n = E.size
+    dE = _device.upload(E.ravel(), device=gpu)
+    flag = (E.ravel() > 0).astype(numpy.int32)
+    uidx = numpy.concatenate([[0], numpy.cumsum(flag)[:-1]]).astype(numpy.int32)
+    d_uidx = _device.upload(uidx, dtype=torch.int32, device=gpu)
+    d_u = _device.upload(uniforms if len(uniforms) else numpy.zeros(1), device=gpu)
+    d_noise = None if noise is None else _device.upload(noise.ravel(), device=gpu)
+    adc = torch.empty(n, dtype=torch.float64, device=gpu)
+    sig = torch.empty(n, dtype=torch.float64, device=gpu)
+    status = torch.zeros(1, dtype=torch.int32, device=gpu)
+    gm = None if gain_map is None else _device.upload(gain_map.ravel(), device=gpu)
+    om = None if offset_map is None else _device.upload(offset_map.ravel(), device=gpu)
+    _lib.call('smx_emccd_parity', n, dE.data_ptr(), _device.ptr(d_noise), d_u.data_ptr(),
+              d_uidx.data_ptr(), _device.ptr(gm), _device.ptr(om), det, -1.0, None,
+              adc.data_ptr(), sig.data_ptr(), status.data_ptr(), _device.stream())
+    return adc.cpu().numpy().reshape(E.shape), sig.cpu().numpy().reshape(E.shape), int(status.item())
+
+
+def _oracle(p, E, noise, uniforms, gain=None, offset=None):
+    S = O.emccd_signal_predrawn(E, p.emgain, uniforms)
+    adc = O.adc_counts(S + noise, p.adc_fullwell, p.adc_gain if gain is None else gain,
+                       p.adc_offset if offset is None else offset, p.adc_bit)
+    return adc, S
+
+
+@pytest.mark.parametrize('seed,scale', [(0, 0.0), (1, 1.0), (2, 10.0), (3, 100.0), (4, 600.0)])
+def test_emccd_parity_bit_exact(gpu, seed, scale):
+    rng = numpy.random.RandomState(seed)
+    shape = (24, 20)
+    p = oracle_params({'detector': {'image_size': list(shape)}})
+    E = 0.0092 + scale * rng.exponential(1.0, size=shape) * (rng.uniform(size=shape) < 0.5)
+    noise = rng.normal(0, p.readout_noise, shape)
+    u = rng.random_sample(int((E > 0).sum()))
+    ref_adc, ref_S = _oracle(p, E, noise, u)
+    adc, S, status = _gpu_parity(gpu, E, noise, u, _det_params(p))
+    assert status == 0
+    assert numpy.array_equal(S, ref_S)
+    assert numpy.array_equal(adc, ref_adc)
+
+
+def test_emccd_parity_zero_and_edge_uniforms(gpu):
+    p = oracle_params({'detector': {'image_size': [4, 4]}})
+    E = numpy.array([[0.0, 0.0092, 0.5, 3.0], [12.0, 0.0, 1e-6, 50.0],
+                     [0.0092, 0.0092, 0.0092, 0.0092], [2.0, 2.0, 2.0, 2.0]])
+    u = numpy.array([0.0, 0.999999999999, 0.5, 0.25, 0.75, 1e-300, 0.9,
+                     0.0, 0.99084, 0.9909, 0.5, 0.1, 0.2, 0.3])[: int((E > 0).sum())]
+    noise = numpy.zeros_like(E)
+    ref_adc, ref_S = _oracle(p, E, noise, u)
+    adc, S, status = _gpu_parity(gpu, E, noise, u, _det_params(p))
+    assert status == 0
+    assert numpy.array_equal(S, ref_S)
+    assert numpy.array_equal(adc, ref_adc)
+
+
+def test_emccd_parity_fpn_maps(gpu):
+    rng = numpy.random.RandomState(3)
+    p = oracle_params({'detector': {'image_size': [16, 16]},
+                       'analog_to_digital_converter': {'type': 'pixel', 'count': 30.0}},
+                      rng=numpy.random.RandomState(8))
+    E = rng.exponential(3.0, (16, 16))
+    noise = rng.normal(0, 100, (16, 16))
+    u = rng.random_sample(256)
+    ref_adc, _ = _oracle(p, E, noise, u)
+    adc, _, status = _gpu_parity(gpu, E, noise, u, _det_params(p), p.adc_gain, p.adc_offset)
+    assert status == 0
+    assert numpy.array_equal(adc, ref_adc)
+
+
+def test_emccd_parity_overflow_is_reported(gpu):
+    # exp(-X + Y) overflows near X = E for E > ~709.8: numpy.random.choice raises there
+    p = oracle_params({'detector': {'image_size': [1, 2]}})
+    E = numpy.array([[5.0, 800.0]])
+    with pytest.raises(ValueError):
+        O.emccd_signal_predrawn(E, p.emgain, numpy.array([0.5, 0.5]))
+    _, _, status = _gpu_parity(gpu, E, None, numpy.array([0.5, 0.5]), _det_params(p))
+    assert status & 4
+
+
+def test_emccd_philox_moments(gpu):
+    """Philox EMCCD: mean g E and variance 2 g^2 E (excess noise factor 2),
+    atom P(S=0) = exp(-E), readout noise N(0, sigma^2)."""
+    from scopyon_amd import _lib, _device
+    p = oracle_params({'detector': {'image_size': [512, 512]}})
+    det = _det_params(p)
+    det.readout_noise = 0.0
+    det.background_switch = 0
+    n = 512 * 512
+    for E_val in (0.05, 2.0, 30.0):
+        photons = torch.full((n,), E_val / p.QE, dtype=torch.float64, device=gpu)
+        adc = torch.empty(n, dtype=torch.float64, device=gpu)
+        E = torch.empty(n, dtype=torch.float64, device=gpu)
+        det.gain, det.offset, det.fullwell, det.adc_max = 1.0, 0.0, 1e300, 1e300
+        _lib.call('smx_emccd_philox', n, photons.data_ptr(), None, None, det, 1234, 7, 0,
+                  E.data_ptr(), adc.data_ptr(), 0, _device.stream())
+        S = adc.cpu().numpy()
+        g = p.emgain
+        Ev = E.cpu().numpy()[0]
+        assert abs(S.mean() - g * Ev) < 5 * numpy.sqrt(2 * g * g * Ev / n)
+        assert abs(S.var() / (2 * g * g * Ev) - 1) < 0.08
+        assert abs((S == 0).mean() - numpy.exp(-Ev)) < 5 * numpy.sqrt(numpy.exp(-Ev) / n) + 1e-4
+        assert (S == numpy.floor(S)).all()

@@ -1,0 +1,75 @@
+"""Pins for the oracle's EMCCD model: the pmf is the compound Poisson-Gamma
+density (evaluated here by its defining series), moments, and the
+RandomState.choice restatement used for pre-drawn uniforms."""
+import math
+
+import numpy
+import pytest
+from scipy import stats
+
+from oracle import epifm as O
+
+
+def _series_density(S, E, g):
+    """sum_{n>=1} Pois(n; E) Gamma(S; n, g), then divided by 2 (E/g) e^{-E}."""
+    out = numpy.zeros(len(S))
+    for n in range(1, 200):
+        out += stats.poisson.pmf(n, E) * stats.gamma.pdf(S, n, scale=g)
+    return out / (2 * (E / g) * math.exp(-E))
+
+
+@pytest.mark.parametrize('E', [0.0092, 0.7, 4.0, 25.0])
+def test_pmf_is_compound_poisson_gamma(E):
+    g = 300
+    S, p = O.emccd_distribution(E, g)
+    a = 1.0 / g
+    raw = O.emccd_probability(S[1:] if S[0] == 0 else S, E, a)
+    ref = _series_density((S[1:] if S[0] == 0 else S).astype(float), E, g)
+    mask = ref > 1e-300
+    numpy.testing.assert_allclose(raw[mask], ref[mask], rtol=1e-9)
+    if S[0] == 0:
+        # the atom relative to the scaled density: e^{-E} / (2 (E/g) e^{-E})
+        total = p[0] / (1.0 / (2 * a * E))
+        assert p[0] * (1.0 / p[0]) == 1.0
+        # normalised atom mass ~ e^{-E}
+        assert p[0] == pytest.approx(math.exp(-E), rel=2e-3 if E < 1 else 5e-2)
+
+
+@pytest.mark.parametrize('E', [0.5, 3.0, 20.0])
+def test_pmf_moments(E):
+    g = 300
+    S, p = O.emccd_distribution(E, g)
+    mean = (S * p).sum()
+    var = (S * S * p).sum() - mean ** 2
+    assert mean == pytest.approx(g * E, rel=2e-3)
+    assert var == pytest.approx(2 * g * g * E, rel=5e-3)
+
+
+def test_choice_restatement_matches_randomstate():
+    """emccd_index(E, g, u) with u = random_sample() == RandomState.choice."""
+    a = numpy.random.RandomState(21)
+    b = numpy.random.RandomState(21)
+    for E in [0.0092, 0.3, 2.0, 9.0, 60.0, 0.0092, 150.0]:
+        ref = O.emccd_draw(E, 300, a)
+        got = O.emccd_index(E, 300, b.random_sample())
+        assert ref == got
+
+
+def test_no_draw_for_nonpositive_expectation():
+    a = numpy.random.RandomState(1)
+    assert O.emccd_draw(0.0, 300, a) == 0
+    assert a.random_sample() == numpy.random.RandomState(1).random_sample()
+
+
+def test_adc_formula():
+    x = numpy.array([-5e5, 0.0, 1000.0, 5e5, 9e5])
+    adc = O.adc_counts(x, 800000, 800000 / 63536, 2000, 16)
+    assert adc[0] == 0 and adc[-1] == min(800000 / (800000 / 63536) + 2000, 65535)
+    assert adc[2] == 1000.0 / (800000 / 63536) + 2000
+
+
+def test_emccd_memo_does_not_change_draws():
+    E = numpy.array([[0.0092, 0.5], [0.0092, 3.0]])
+    a = O.emccd_signal(E, 300, numpy.random.RandomState(4), memo=True)
+    b = O.emccd_signal(E, 300, numpy.random.RandomState(4), memo=False)
+    assert numpy.array_equal(a, b)
