@@ -1,0 +1,249 @@
+"""Benchmark: batched TIRF/EMCCD frame simulation on one MI355X.
+
+Workload (the metric's configuration): F fields of view, each a 512 x 512
+EMCCD frame (16 um pixels, 100x magnification -> 160 nm) with 10,000 emitters
+diffusing in 2D (D = 0.1 um^2/s, dt = 33 ms, reflecting walls at the frame
+edge), TRITC Born-Wolf PSF integrated per pixel from the float64 summed-area
+table, EM gain 300, readout sigma 100 e-, 16-bit ADC.  One step = Brownian step
++ emitter weights + splat + EMCCD/ADC for every frame of the batch, replayed
+from one captured HIP graph.  Inputs are synthetic and resident in HBM.
+
+Also reported: the plain single-frame path (same kernels, one frame per step,
+eager launches), the drop-in form_image() path (RandomState stream,
+reference-exact EMCCD), per-stage kernel times from HIP events, the roofline of
+the dominant kernel, and the CPU baseline (the float64 oracle on this host).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--fovs F]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("frames/sec (512², 10k emitters) batched vs plain at ≤1e-5 err; "
+          "fwd+bwd/sec")
+HBM_PEAK_GBS = 8000.0
+
+
+def _config(size, exposure):
+    import scopyon_amd
+    cfg = scopyon_amd.DefaultConfiguration()
+    cfg.update({'default': {'magnification': 100.0,
+                            'detector': {'image_size': [size, size], 'exposure_time': exposure},
+                            'effects': {'photo_bleaching': {'switch': False}}}})
+    return cfg
+
+
+def _stage_times(sim, reps):
+    """Average duration (ms) of each stage of an eager step, from HIP events
+    on the launch stream."""
+    import torch
+    stages = {'brownian_philox': sim.move, 'weights+splat': sim.form, 'emccd_philox': sim.detect}
+    out = {}
+    for name, fn in stages.items():
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for a, b in evs:
+            a.record()
+            fn()
+            b.record()
+        torch.cuda.synchronize()
+        out[name] = sum(a.elapsed_time(b) for a, b in evs) / reps
+    return out
+
+
+def _splat_only_time(sim, reps):
+    """Average duration (ms) of the splat call alone (binning + tile kernel)."""
+    import torch
+    from scopyon_amd import _lib, _device
+    t = sim.tables
+    st = _device.stream()
+
+    def splat():
+        _lib.call('smx_splat_expected', sim.n, sim.u.data_ptr(), sim.v.data_ptr(),
+                  sim.w.data_ptr(), sim.key.data_ptr(), sim.fov_id.data_ptr(),
+                  sim.fovs_dev.data_ptr(), sim.nfov, sim.n_tiles, t.sat.data_ptr(), t.sat_elems,
+                  len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, sim.capacity, 1,
+                  _device.ptr(t.K if sim.use_phase else None), t.m, sim.photons.data_ptr(),
+                  sim.ws.data_ptr(), sim.ws.numel(), sim.splatter.status.data_ptr(), st)
+    splat()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        splat()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def cpu_baseline(size, n_emit, sample_emitters, seed=1):
+    """Time the float64 oracle (single core) on a bounded sample of one frame:
+    `sample_emitters` emitters splatted + the full per-pixel EMCCD loop; the
+    splat time is scaled to `n_emit` emitters.  Also returns the GPU-vs-oracle
+    relative error of the noise-free image on the same sample."""
+    import numpy
+    import torch
+    from oracle import epifm as O, params as P
+    import scopyon_amd
+    from scopyon_amd import _epifm
+    ocfg = P.default_config()
+    ocfg['magnification'] = 100.0
+    ocfg['detector']['image_size'] = [size, size]
+    ocfg['detector']['exposure_time'] = 0.033
+    p = P.epifm_params(ocfg, numpy.random.RandomState(0))
+    rng = numpy.random.RandomState(seed)
+    pl = 16e-6 / 100.0
+    half = 0.5 * size * pl
+    xy = rng.uniform(-half, half, size=(sample_emitters, 2))
+    parts = numpy.column_stack([numpy.zeros(sample_emitters), xy, numpy.arange(sample_emitters),
+                                numpy.ones(sample_emitters)])
+    psf = O.PSFCache(p)
+    psf.get(0.0)    # table build is a one-time cost, not per frame
+    t0 = time.perf_counter()
+    exp, _, _ = O.molecule_plane(p, psf, parts, (size, size), numpy.zeros(3), 0.033, {}, None,
+                                 None)
+    t_splat = time.perf_counter() - t0
+    cam = numpy.zeros((size, size, 2))
+    cam[:, :, 0] = exp
+    t0 = time.perf_counter()
+    photons = cam[:, :, 0] + p.background_mean
+    E = p.QE * photons
+    noise = O.emccd_noise(E.shape, p.readout_noise, rng)
+    sig = O.emccd_signal(E, p.emgain, rng, memo=False)
+    O.adc_counts(sig + noise, p.adc_fullwell, p.adc_gain, p.adc_offset, p.adc_bit)
+    t_det = time.perf_counter() - t0
+    per_frame = t_splat * (n_emit / sample_emitters) + t_det
+    # GPU noise-free image on the same particles
+    cfg = _config(size, 0.033)
+    sim = _epifm._EPIFMSimulator(_epifm.EPIFMConfigs(cfg.default, rng=numpy.random.RandomState(0)))
+    camera = torch.zeros(size * size, dtype=torch.float64, device=sim.device)
+    sim._substep(parts, 0.033, camera, None, None, {}, False)
+    got = camera.cpu().numpy().reshape(size, size)
+    Eg = p.QE * (got + p.background_mean)
+    rel = float(numpy.max(numpy.abs(Eg - E) / numpy.abs(E)))
+    return {'value': 1.0 / per_frame, 'unit': 'frames/s', 'cores': 1, 'kind': 'port',
+            'sample': ('one {0}x{0} frame: {1} emitters splatted (scaled x{2:g} to {3}) + '
+                       'per-pixel EMCCD over all {4} pixels, float64 numpy/scipy oracle, '
+                       'single core'.format(size, sample_emitters, n_emit / sample_emitters,
+                                            n_emit, size * size)),
+            'seconds': {'splat_sample': t_splat, 'detector': t_det}}, rel
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--fovs', type=int, default=64)
+    ap.add_argument('--emitters', type=int, default=10000)
+    ap.add_argument('--size', type=int, default=512)
+    ap.add_argument('--plain-steps', type=int, default=20)
+    ap.add_argument('--cpu-sample', type=int, default=200)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only; --gpus {} is not supported (the FOV batch '
+              'would shard across GPUs with no communication, see DESIGN.md)'.format(args.gpus),
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy
+    import torch
+    import scopyon_amd
+    from scopyon_amd.batched import BatchedSimulator
+
+    dt = 0.033
+    cfg = _config(args.size, dt)
+    F, S = args.fovs, args.size
+    sim = BatchedSimulator(cfg, [(S, S)] * F, args.emitters, D=0.1e-12, dt=dt, seed=1234,
+                           boundary='reflect')
+    sim.capture()
+    for _ in range(args.warmup):
+        sim.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.replay(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert sim.status() & ~1 == 0
+    batched_fps = F * args.steps / elapsed
+
+    # plain single-frame path: same kernels, one frame per step, eager launches
+    plain = BatchedSimulator(cfg, [(S, S)], args.emitters, D=0.1e-12, dt=dt, seed=99,
+                             boundary='reflect')
+    for _ in range(3):
+        plain.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        plain.step()
+    torch.cuda.synchronize()
+    plain_fps = args.plain_steps / (time.perf_counter() - t0)
+
+    # drop-in API: form_image with a RandomState (reference-exact EMCCD draws)
+    rng = numpy.random.RandomState(5)
+    pl = 16e-6 / 100.0
+    inputs = rng.uniform(-0.5 * S * pl, 0.5 * S * pl, size=(args.emitters, 2))
+    scopyon_amd.form_image(inputs, config=cfg, rng=rng)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    napi = 3
+    for _ in range(napi):
+        scopyon_amd.form_image(inputs, config=cfg, rng=rng)
+    api_fps = napi / (time.perf_counter() - t0)
+
+    # per-stage times (HIP events on the launch stream) and the dominant kernel
+    stages = _stage_times(sim, 10)
+    splat_ms = _splat_only_time(sim, 10)
+    N, HW = sim.n, sim.n_pix
+    stage_bytes = {
+        # coords read+write (2 x 16 B) + fov_id 4 B + box per emitter
+        'brownian_philox': N * 36,
+        # depth read 8 B, weight 8 B + key 4 B written; splat re-reads u, v, w, key
+        # (28 B) and read-modify-writes every float64 pixel (16 B)
+        'weights+splat': N * 20 + N * 28 + HW * 16,
+        # photons read 8 B, float32 ADC written 4 B
+        'emccd_philox': HW * 12,
+    }
+    dom = max(stages, key=stages.get)
+    achieved = stage_bytes[dom] / (stages[dom] * 1e-3) / 1e9
+    result = {
+        'metric': METRIC,
+        'value': batched_fps,
+        'unit': 'frames/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': elapsed / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': 'batched TIRF EMCCD frames, {0}x{0} px, {1} emitters per FOV, '
+                               '{2} FOVs per step, hipGraph replay'.format(S, args.emitters, F),
+                   'fovs': F, 'frame': [S, S], 'emitters_per_fov': args.emitters,
+                   'pixel_nm': 160, 'psf': 'TRITC Born-Wolf, float64 SAT', 'dt_s': dt,
+                   'D_um2_s': 0.1, 'walls': 'reflecting', 'emccd': 'philox'},
+        'plain_frames_per_sec': plain_fps,
+        'batched_over_plain': batched_fps / plain_fps,
+        'api_form_image_frames_per_sec': api_fps,
+        'stage_ms': stages,
+        'splat_call_ms': splat_ms,
+        'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'algorithmic_bytes': stage_bytes[dom]},
+    }
+    if not args.no_cpu:
+        base, rel = cpu_baseline(S, args.emitters, args.cpu_sample)
+        result['cpu_baseline'] = base
+        result['max_rel_err_vs_fp64_oracle'] = rel
+        result['gpu_over_cpu'] = batched_fps / base['value']
+    print(json.dumps(result))
+
+
+if __name__ == '__main__':
+    main()

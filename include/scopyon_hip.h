@@ -76,13 +76,24 @@ int smx_emitter_weights(int n, const double* depth, const double* p_state,
 /* Tiles of a frame (16 x 16 pixels) and the splat workspace size. */
 int smx_splat_tiles(const smx_fov_t* fov /* host */, int reserved);
 int smx_splat_tile_size(void);
-size_t smx_splat_workspace_bytes(int n_tiles, long long list_capacity);
+size_t smx_splat_workspace_bytes(int n_emitters, int n_tiles, long long list_capacity);
+/* Phase tables for pixel lengths of `sat_period` (integer >= 2) table cells:
+ * K[slot][phi_u][phi_v][du][dv] = photons of footprint pixel (du, dv) for
+ * boundary phases (phi_u, phi_v): the SAT integral times unit_area, 0 unless
+ * positive; m = footprint pixels per axis.  Lets smx_splat_expected integrate
+ * each (emitter, pixel) with one load. */
+size_t smx_phase_table_elems(int sat_period, int m);
+int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int ntab,
+                          int sat_period, int m, double unit_area, double* K,
+                          hipStream_t stream);
 /* Pixel integration of n emitters into nfov frames: replaces
  * PointSpreadingFunction.overlay_signal_ called per emitter
  * (_epifm.py:223-282 via _epifm.py:1261-1264).  u, v: absolute lateral
  * coordinates; w: weights; key: SAT slot per emitter; fov_id: FOV per emitter
  * (NULL: all in FOV 0).  sat: tables of sat_stride doubles each.
  * accumulate=1 adds into `out` (camera += expected); 0 overwrites.
+ * K (may be NULL): phase tables from smx_phase_table_build for the same slots
+ * (identical results, one load per emitter-pixel instead of four).
  * status bit 1: a tile list exceeded the LDS sort capacity (still correct,
  * order not by emitter index); bit 2: list capacity too small (result
  * incomplete); bit 8: an emitter's table slot was not built (skipped). */
@@ -90,8 +101,8 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
                        const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                        double res, double unit_area, long long list_capacity, int accumulate,
-                       double* out, void* workspace, size_t workspace_bytes, int* status_dev,
-                       hipStream_t stream);
+                       const double* K, int m, double* out, void* workspace,
+                       size_t workspace_bytes, int* status_dev, hipStream_t stream);
 
 /* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */
 /* E = QE * (photons + background) (_epifm.py:1436-1441); flag_out (may be
@@ -115,11 +126,13 @@ int smx_emccd_parity(int n, const double* E, const double* noise, const double* 
                      const double* T_common, double* adc_out, double* signal_out,
                      int* status_dev, hipStream_t stream);
 /* EMCCD with Philox4x32-10 randomness: Poisson(E) -> Gamma(n, emgain) signal,
- * Gaussian readout noise, ADC.  Fuses _epifm.py:1436-1469. */
+ * Gaussian readout noise, ADC.  Fuses _epifm.py:1436-1469.  The Philox frame
+ * counter is `frame` plus *frame_dev when frame_dev is not NULL. */
 int smx_emccd_philox(int n, const double* photons, const double* gain_map,
                      const double* offset_map, const smx_detector_params_t* params /* host */,
-                     uint64_t seed, uint32_t frame, uint32_t pix_base, double* E_out,
-                     void* adc_out, int adc_is_float32, hipStream_t stream);
+                     uint64_t seed, uint32_t frame, uint32_t pix_base,
+                     const uint32_t* frame_dev, double* E_out, void* adc_out,
+                     int adc_is_float32, hipStream_t stream);
 
 /* ---- Brownian sampling (sampling2.py:33-68, sampling.py:88-119) --------- */
 /* Whole trajectories from caller-drawn normals xi[t][i][d] (and transition
@@ -132,11 +145,17 @@ int smx_brownian_predrawn(int n, int ndim, int nsteps, int nstates, const double
                           const double* lower, const double* upper, const double* pacc,
                           const double* trans_u, double* xs, int* states, hipStream_t stream);
 /* One Philox step on structure-of-arrays coordinates: x += sqrt(2 D dt) N(0,1);
- * boundary 0 none, 1 periodic, 2 reflecting.  lower/upper: host, ndim each. */
+ * boundary 0 none, 1 periodic, 2 reflecting.  lower/upper: host, ndim each;
+ * or, with fov_box (device, 6 doubles per FOV: lo[3], hi[3]) and fov_id,
+ * per-FOV walls.  Extension of sampling2.py:33-50 (the reference has only
+ * the periodic wrap). */
 int smx_brownian_philox(int n, int ndim, double* x0, double* x1, double* x2, const double* D,
                         double D_scalar, double dt, int boundary, const double* lower,
                         const double* upper, uint64_t seed, uint32_t step, uint32_t id_base,
-                        hipStream_t stream);
+                        const int* fov_id, int nfov, const double* fov_box,
+                        const uint32_t* step_dev, hipStream_t stream);
+/* counter[0] += by on the device (advances Philox counters inside graphs). */
+int smx_counter_add(uint32_t* counter, uint32_t by, hipStream_t stream);
 /* Uniform initial positions (sampling2.py:15-31) from Philox; lower/upper host. */
 int smx_uniform_init(int n, int ndim, double* x0, double* x1, double* x2, const double* lower,
                      const double* upper, uint64_t seed, uint32_t seq, uint32_t id_base,

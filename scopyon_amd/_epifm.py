@@ -256,6 +256,7 @@ class Splatter:
         self.device = device
         self.wparams = configs.weight_params(tables)
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
+        self.use_phase = True   # phase-table splat when the pixel is an integer number of cells
 
     def weights(self, depth, p_state, unit_time, budget=None, budget_slot=None, slot_map=None):
         n = depth.numel()
@@ -273,13 +274,14 @@ class Splatter:
         n = u.numel()
         t = self.tables
         cap = max(1, n * tiles_per_emitter(self.c.pixel_length, t.ntab))
-        nbytes = lib.smx_splat_workspace_bytes(n_tiles, cap)
+        nbytes = lib.smx_splat_workspace_bytes(n, n_tiles, cap)
         ws = _device.ARENA.get('splat_ws', nbytes, self.device)
+        K = t.K if (t.use_phase and self.use_phase) else None
         _lib.call('smx_splat_expected', n, _device.ptr(u), _device.ptr(v), _device.ptr(w),
                   _device.ptr(key), _device.ptr(fov_id), fovs_dev.data_ptr(), nfov, n_tiles,
                   t.sat.data_ptr(), t.sat_elems, len(t.slot_of), t.ntab, t.period, RES, RES * RES,
-                  cap, 1 if accumulate else 0, out.data_ptr(), ws.data_ptr(), ws.numel(),
-                  self.status.data_ptr(), _device.stream())
+                  cap, 1 if accumulate else 0, _device.ptr(K), t.m, out.data_ptr(),
+                  ws.data_ptr(), ws.numel(), self.status.data_ptr(), _device.stream())
 
 
 def _status_check(status):
@@ -411,7 +413,7 @@ class _EPIFMSimulator:
             frame = self._philox_frame
             self._philox_frame += 1
             _lib.call('smx_emccd_philox', n, camera.data_ptr(), _device.ptr(gain_map),
-                      _device.ptr(offset_map), self.det, rng.seed, frame, 0, E.data_ptr(),
+                      _device.ptr(offset_map), self.det, rng.seed, frame, 0, None, E.data_ptr(),
                       adc.data_ptr(), 0, st)
             return E, adc
         lib = _lib.lib()

@@ -61,10 +61,12 @@ SIGNATURES = {
                                     P, c_int, P, P, P, P]),
     'smx_splat_tiles': (c_int, [ctypes.POINTER(FovT), c_int]),
     'smx_splat_tile_size': (c_int, []),
-    'smx_splat_workspace_bytes': (c_size_t, [c_int, c_longlong]),
+    'smx_splat_workspace_bytes': (c_size_t, [c_int, c_int, c_longlong]),
+    'smx_phase_table_elems': (c_size_t, [c_int, c_int]),
+    'smx_phase_table_build': (c_int, [P, c_size_t, c_int, c_int, c_int, c_int, c_double, P, P]),
     'smx_splat_expected': (c_int, [c_int, P, P, P, P, P, P, c_int, c_int, P, c_size_t, c_int,
-                                   c_int, c_int, c_double, c_double, c_longlong, c_int, P, P,
-                                   c_size_t, P, P]),
+                                   c_int, c_int, c_double, c_double, c_longlong, c_int, P, c_int,
+                                   P, P, c_size_t, P, P]),
     'smx_detector_expectation': (c_int, [c_int, P, ctypes.POINTER(DetectorParamsT), P, P, P]),
     'smx_scan_workspace_bytes': (c_size_t, [c_int]),
     'smx_exclusive_scan_i32': (c_int, [c_int, P, P, P, c_size_t, P]),
@@ -72,11 +74,12 @@ SIGNATURES = {
     'smx_emccd_parity': (c_int, [c_int, P, P, P, P, P, P, ctypes.POINTER(DetectorParamsT),
                                  c_double, P, P, P, P, P]),
     'smx_emccd_philox': (c_int, [c_int, P, P, P, ctypes.POINTER(DetectorParamsT), c_uint64,
-                                 c_uint32, c_uint32, P, P, c_int, P]),
+                                 c_uint32, c_uint32, P, P, P, c_int, P]),
     'smx_brownian_predrawn': (c_int, [c_int, c_int, c_int, c_int, P, P, P, P, c_int, P, P, P, P,
                                       P, P, P]),
     'smx_brownian_philox': (c_int, [c_int, c_int, P, P, P, P, c_double, c_double, c_int, P, P,
-                                    c_uint64, c_uint32, c_uint32, P]),
+                                    c_uint64, c_uint32, c_uint32, P, c_int, P, P, P]),
+    'smx_counter_add': (c_int, [P, c_uint32, P]),
     'smx_uniform_init': (c_int, [c_int, c_int, P, P, P, P, P, c_uint64, c_uint32, c_uint32, P]),
 }
 

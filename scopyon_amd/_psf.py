@@ -56,12 +56,19 @@ class PSFTables:
         self.alpha = k * na
         self.gamma = k * numpy.power(na / 2, 2)
         self.scale = self.alpha * self.alpha / numpy.pi
-        # key (0..n_keys-2 for depth keys, n_keys-1 for the cutoff key) -> slot
-        self.n_keys = int(depth_cutoff / RES) + 2
+        # key index: 0..n_keys-2 for depth keys (int(|d| / res) for |d| < cutoff + res,
+        # with one key of margin), n_keys-1 for the cutoff key
+        self.n_keys = int(round(depth_cutoff / RES)) + 3
         self.cutoff_index = self.n_keys - 1
         self.slot_of = {}
         self.capacity = 0
         self.sat = None
+        # phase tables (integer pixel/cell ratio only): footprint pixels per axis
+        self.m = int(math.ceil(RES * (self.ntab - 1) / pixel_length)) + 2 if pixel_length else 0
+        self.use_phase = self.period > 1
+        self.k_elems = (int(_lib.lib().smx_phase_table_elems(self.period, self.m))
+                        if self.use_phase else 0)
+        self.K = None
 
     def key_index(self, key):
         return self.cutoff_index if key < 0 else int(key)
@@ -76,6 +83,11 @@ class PSFTables:
         new = torch.empty((cap, self.sat_elems), dtype=torch.float64, device=self.device)
         if self.sat is not None and self.capacity:
             new[: self.capacity].copy_(self.sat[: self.capacity])
+        if self.use_phase:
+            newk = torch.empty((cap, self.k_elems), dtype=torch.float64, device=self.device)
+            if self.K is not None and self.capacity:
+                newk[: self.capacity].copy_(self.K[: self.capacity])
+            self.K = newk
         self.sat, self.capacity = new, cap
 
     def ensure(self, keys):
@@ -107,6 +119,9 @@ class PSFTables:
                                    self.device)
             _lib.call('smx_psf_sat_build', radial[s:s + m].data_ptr(), m, self.nr, RES,
                       self.period, ws.data_ptr(), self.sat[base + s].data_ptr(), st)
+        if self.use_phase:
+            _lib.call('smx_phase_table_build', self.sat[base].data_ptr(), self.sat_elems, nz,
+                      self.ntab, self.period, self.m, RES * RES, self.K[base].data_ptr(), st)
         for i, k in enumerate(missing):
             self.slot_of[k] = base + i
         self.radial_last = radial
@@ -122,5 +137,5 @@ class PSFTables:
 
     def prebuild_all(self):
         """Build every depth key (0 .. cutoff) plus the cutoff key."""
-        keys = list(range(self.n_keys - 1)) + [-1]
+        keys = list(range(self.n_keys - 2)) + [-1]
         self.ensure(keys)

@@ -1,0 +1,193 @@
+"""Many fields of view (FOVs) simulated together on one GPU.
+
+Each FOV is an independent camera frame (its own size and emitter count) that
+shares the optics/detector configuration.  One time step of the whole batch is
+one fixed launch sequence -- Philox Brownian step with per-FOV walls, emitter
+weights, tile-binned splat over every frame, Philox EMCCD + ADC over every
+pixel -- so it can be captured once in a HIP graph and replayed
+(``capture()``/``replay()``).  Philox counters live in device memory and advance
+inside the graph, so every replay draws fresh numbers.
+
+Layout in HBM (structure of arrays): emitter coordinates ``depth, u, v``
+(float64, concatenated FOV after FOV, CSR ``emitter_offsets``), ``fov_id``
+(int32 per emitter), per-emitter ``D``; frames concatenated at
+``pix_off`` offsets: photons/expectation (float64) and ADC counts (float32 by
+default).  The PSF summed-area tables are shared by all FOVs.
+"""
+import numpy
+import torch
+
+from . import _device, _lib
+from ._epifm import EPIFMConfigs, FrameGeometry, Splatter, tiles_per_emitter
+from ._psf import PSFTables
+
+__all__ = ['BatchedSimulator']
+
+
+class BatchedSimulator:
+
+    def __init__(self, config, fov_shapes, n_emitters, *, D=0.1e-12, dt=None, seed=0,
+                 boundary='reflect', box=None, depth_range=None, method='default',
+                 adc_dtype=torch.float32, keep_expectation=False):
+        self.device = _device.require_gpu()
+        conf = config[method] if hasattr(config, '__getitem__') else config
+        self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
+        if not c.uniform_adc():
+            raise NotImplementedError('fixed-pattern ADC maps are not supported in batches')
+        if c.detector_type != 'EMCCD':
+            raise NotImplementedError('batched simulation supports the EMCCD detector')
+        self.dt = float(dt if dt is not None else c.detector_exposure_time)
+        self.seed = int(seed)
+        self.nfov = len(fov_shapes)
+        counts = numpy.broadcast_to(numpy.asarray(n_emitters, dtype=numpy.int64), (self.nfov,))
+        self.emitter_offsets = numpy.concatenate([[0], numpy.cumsum(counts)])
+        self.n = int(self.emitter_offsets[-1])
+        pl = c.pixel_length
+        p0 = c.detector_focal_point
+        geoms, pix, tiles = [], 0, 0
+        for (nw, nh) in fov_shapes:
+            g = FrameGeometry(nw, nh, pl, p0[1], p0[2], pix_off=pix, tile_off=tiles)
+            geoms.append(g)
+            pix += int(nw) * int(nh)
+            tiles += g.n_tiles
+        self.geoms = geoms
+        self.n_pix, self.n_tiles = pix, tiles
+        self.pix_offsets = numpy.array([g.fov.pix_off for g in geoms] + [pix])
+        raw = b''.join(bytes(g.fov) for g in geoms)
+        self.fovs_dev = torch.from_numpy(numpy.frombuffer(raw, dtype=numpy.uint8).copy()).to(
+            self.device)
+        # PSF tables: depth 0 only for 2D batches, every key for 3D batches
+        self.tables = PSFTables(c.fluorophore_type, c.psf_wavelength, c.radial_cutoff,
+                                c.depth_cutoff, c.psf_radial_width, pl, self.device)
+        if depth_range is None:
+            self.tables.ensure([0])
+        else:
+            self.tables.prebuild_all()
+        self.slot_map = self.tables.slot_map()
+        self.splatter = Splatter(c, self.tables, self.device)
+        self.det = c.detector_params()
+        self.boundary = {'none': 0, 'periodic': 1, 'reflect': 2}[boundary]
+        # emitters: uniform in each FOV's box (default: the frame extent)
+        hostrng = numpy.random.Generator(numpy.random.Philox(self.seed))
+        depth = numpy.zeros(self.n)
+        u = numpy.empty(self.n)
+        v = numpy.empty(self.n)
+        boxes = numpy.zeros((self.nfov, 6))
+        fov_id = numpy.empty(self.n, dtype=numpy.int32)
+        for f, g in enumerate(geoms):
+            a, b = self.emitter_offsets[f], self.emitter_offsets[f + 1]
+            if box is None:
+                hu, hv = g.fov.half_w, g.fov.half_h
+                lo = [p0[0], p0[1] - hu, p0[2] - hv]
+                hi = [p0[0], p0[1] + hu, p0[2] + hv]
+            else:
+                lo = [p0[0], p0[1] - box[0] / 2, p0[2] - box[1] / 2]
+                hi = [p0[0], p0[1] + box[0] / 2, p0[2] + box[1] / 2]
+            if depth_range is not None:
+                lo[0], hi[0] = p0[0] + depth_range[0], p0[0] + depth_range[1]
+                depth[a:b] = hostrng.uniform(lo[0], hi[0], b - a)
+            else:
+                depth[a:b] = p0[0]
+            u[a:b] = hostrng.uniform(lo[1], hi[1], b - a)
+            v[a:b] = hostrng.uniform(lo[2], hi[2], b - a)
+            boxes[f] = lo + hi
+            fov_id[a:b] = f
+        self.ndim = 3 if depth_range is not None else 2
+        self.depth = _device.upload(depth, device=self.device)
+        self.u = _device.upload(u, device=self.device)
+        self.v = _device.upload(v, device=self.device)
+        self.fov_id = _device.upload(fov_id, dtype=torch.int32, device=self.device)
+        # walls in (depth, u, v) order for the 3D step, (u, v) for the 2D one
+        if self.ndim == 2:
+            z = numpy.zeros(self.nfov)
+            boxes = numpy.column_stack([boxes[:, 1], boxes[:, 2], z, boxes[:, 4], boxes[:, 5], z])
+        self.fov_box = _device.upload(numpy.ascontiguousarray(boxes), device=self.device)
+        if numpy.ndim(D) == 0:
+            self.D = None
+            self.D_scalar = float(D)
+        else:
+            self.D = _device.upload(numpy.asarray(D, dtype=float), device=self.device)
+            self.D_scalar = 0.0
+        self.p_state = None
+        # frame buffers and device counters
+        self.photons = torch.zeros(self.n_pix, dtype=torch.float64, device=self.device)
+        self.adc = torch.zeros(self.n_pix, dtype=adc_dtype, device=self.device)
+        self.E = (torch.zeros(self.n_pix, dtype=torch.float64, device=self.device)
+                  if keep_expectation else None)
+        self.w = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        self.key = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
+        cap = max(1, self.n * tiles_per_emitter(pl, self.tables.ntab))
+        self.capacity = cap
+        self.ws = torch.empty(_lib.lib().smx_splat_workspace_bytes(self.n, self.n_tiles, cap),
+                              dtype=torch.uint8, device=self.device)
+        self.use_phase = self.tables.use_phase
+        self.graph = None
+
+    # ---- one step ----
+    def move(self):
+        st = _device.stream()
+        xs = [self.depth, self.u, self.v] if self.ndim == 3 else [self.u, self.v, self.u]
+        _lib.call('smx_brownian_philox', self.n, self.ndim, xs[0].data_ptr(), xs[1].data_ptr(),
+                  xs[2].data_ptr(), _device.ptr(self.D), self.D_scalar, self.dt, self.boundary,
+                  None, None, self.seed, 0, 0, self.fov_id.data_ptr(), self.nfov,
+                  self.fov_box.data_ptr(), self.counter.data_ptr(), st)
+
+    def form(self):
+        """Noise-free expectation of every frame (photons per pixel before QE/background)."""
+        st = _device.stream()
+        self.photons.zero_()
+        _lib.call('smx_emitter_weights', self.n, self.depth.data_ptr(), None,
+                  self.splatter.wparams, self.dt, None, None, self.slot_map.data_ptr(),
+                  self.slot_map.numel(), self.w.data_ptr(), self.key.data_ptr(), None, st)
+        t = self.tables
+        _lib.call('smx_splat_expected', self.n, self.u.data_ptr(), self.v.data_ptr(),
+                  self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
+                  self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.sat.data_ptr(),
+                  t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 1,
+                  _device.ptr(t.K if self.use_phase else None), t.m, self.photons.data_ptr(),
+                  self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(), st)
+
+    def detect(self):
+        st = _device.stream()
+        _lib.call('smx_emccd_philox', self.n_pix, self.photons.data_ptr(), None, None, self.det,
+                  self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
+                  self.adc.data_ptr(), 1 if self.adc.dtype == torch.float32 else 0, st)
+        _lib.call('smx_counter_add', self.counter.data_ptr(), 1, st)
+
+    def step(self):
+        """Advance every FOV by one time step and form + detect its frame."""
+        self.move()
+        self.form()
+        self.detect()
+
+    # ---- HIP graph ----
+    def capture(self):
+        """Capture one step in a HIP graph (static buffers, device counters)."""
+        self.step()          # warm: workspace, tables, code objects
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.step()
+        self.graph = g
+        return g
+
+    def replay(self, steps=1):
+        for _ in range(steps):
+            if self.graph is None:
+                self.step()
+            else:
+                self.graph.replay()
+
+    # ---- outputs ----
+    def frame(self, f, which='adc'):
+        buf = {'adc': self.adc, 'photons': self.photons, 'expectation': self.E}[which]
+        a, b = self.pix_offsets[f], self.pix_offsets[f + 1]
+        g = self.geoms[f].fov
+        return buf[a:b].view(g.nw, g.nh)
+
+    def status(self):
+        s = int(self.splatter.status.item())
+        if s & 2:
+            raise RuntimeError('splat tile list capacity exceeded')
+        return s

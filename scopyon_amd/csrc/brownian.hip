@@ -76,9 +76,13 @@ __global__ void brownian_philox_kernel(int n, int ndim, double* __restrict__ x0,
                                        const double* __restrict__ D, double D_scalar, double dt,
                                        int boundary, double lo0, double lo1, double lo2,
                                        double hi0, double hi1, double hi2, uint64_t seed,
-                                       uint32_t step, uint32_t id_base) {
+                                       uint32_t step, uint32_t id_base,
+                                       const int* __restrict__ fov_id, int nfov,
+                                       const double* __restrict__ fov_box,
+                                       const uint32_t* __restrict__ step_dev) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (step_dev) step += *step_dev;
     double Di = D ? D[i] : D_scalar;
     double sc = sqrt(2.0 * Di * dt);
     smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u},
@@ -98,6 +102,13 @@ __global__ void brownian_philox_kernel(int n, int ndim, double* __restrict__ x0,
     }
     double* xs[3] = {x0, x1, x2};
     double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    if (fov_box) {
+        // per-FOV walls: fov_box[f] = {lo0, lo1, lo2, hi0, hi1, hi2}
+        int f = fov_id ? fov_id[i] : 0;
+        f = f < 0 ? 0 : (f >= nfov ? nfov - 1 : f);
+        const double* b = fov_box + (size_t)f * 6;
+        for (int d = 0; d < 3; ++d) { lo[d] = b[d]; hi[d] = b[3 + d]; }
+    }
     for (int d = 0; d < ndim; ++d) {
         double v = xs[d][i] + sc * z[d];
         if (boundary == 1) {
@@ -153,21 +164,22 @@ int smx_brownian_predrawn(int n, int ndim, int nsteps, int nstates, const double
 int smx_brownian_philox(int n, int ndim, double* x0, double* x1, double* x2, const double* D,
                         double D_scalar, double dt, int boundary, const double* lower,
                         const double* upper, uint64_t seed, uint32_t step, uint32_t id_base,
-                        hipStream_t stream) {
+                        const int* fov_id, int nfov, const double* fov_box,
+                        const uint32_t* step_dev, hipStream_t stream) {
     if (n < 0 || ndim < 1 || ndim > 3 || boundary < 0 || boundary > 2 ||
         (n > 0 && (!x0 || (ndim > 1 && !x1) || (ndim > 2 && !x2))) ||
-        (boundary && (!lower || !upper))) {
+        (boundary && !fov_box && (!lower || !upper)) || (fov_box && nfov < 1)) {
         smx::set_error("smx_brownian_philox: invalid arguments (n=%d, ndim=%d, boundary=%d)", n,
                        ndim, boundary);
         return SMX_EVALUE;
     }
     if (n == 0) return SMX_OK;
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    if (boundary)
+    if (boundary && !fov_box)
         for (int d = 0; d < ndim; ++d) { lo[d] = lower[d]; hi[d] = upper[d]; }
     hipLaunchKernelGGL(brownian_philox_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n,
                        ndim, x0, x1, x2, D, D_scalar, dt, boundary, lo[0], lo[1], lo[2], hi[0],
-                       hi[1], hi[2], seed, step, id_base);
+                       hi[1], hi[2], seed, step, id_base, fov_id, nfov, fov_box, step_dev);
     SMX_CHECK_LAUNCH("brownian_philox_kernel");
     return SMX_OK;
 }
@@ -186,6 +198,20 @@ int smx_uniform_init(int n, int ndim, double* x0, double* x1, double* x2, const 
     hipLaunchKernelGGL(uniform_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, ndim,
                        x0, x1, x2, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], seed, seq, id_base);
     SMX_CHECK_LAUNCH("uniform_init_kernel");
+    return SMX_OK;
+}
+
+__global__ void counter_add_kernel(uint32_t* c, uint32_t by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c[0] += by;
+}
+
+int smx_counter_add(uint32_t* counter, uint32_t by, hipStream_t stream) {
+    if (!counter) {
+        smx::set_error("smx_counter_add: null counter");
+        return SMX_EVALUE;
+    }
+    hipLaunchKernelGGL(counter_add_kernel, dim3(1), dim3(64), 0, stream, counter, by);
+    SMX_CHECK_LAUNCH("counter_add_kernel");
     return SMX_OK;
 }
 

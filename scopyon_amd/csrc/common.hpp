@@ -123,6 +123,15 @@ SMX_HD double np_remainder(double a, double b) {
 // Ceil of a double, returned as a 64-bit integer (numpy.ceil(..).astype(int)).
 SMX_HD long long ceil_i64(double x) { return (long long)ceil(x); }
 
+// Read through a pointer known to address global memory.  Pointers fetched
+// from by-value argument structs are otherwise treated as generic (flat)
+// pointers, and flat loads share the LDS wait counter, so every LDS wait
+// would also drain the loads in flight.
+template <typename T>
+__device__ inline T gload(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)(p);
+}
+
 // Summed-area-table index for a (side x side) corner grid stored either
 // row-major (period == 1) or phase-major with the given period: corners whose
 // row/column indices share residues mod `period` are contiguous, so one
@@ -135,6 +144,15 @@ struct SatLayout {
         if (period == 1) return (size_t)a * side + b;
         int pa = a % period, pb = b % period;
         return ((size_t)(pa * period + pb) * g + (a / period)) * g + (b / period);
+    }
+    // index(a, b) == row_part(a) + col_part(b)
+    SMX_HD size_t row_part(int a) const {
+        if (period == 1) return (size_t)a * side;
+        return (size_t)(a % period) * period * g * g + (size_t)(a / period) * g;
+    }
+    SMX_HD size_t col_part(int b) const {
+        if (period == 1) return (size_t)b;
+        return (size_t)(b % period) * g * g + (size_t)(b / period);
     }
     SMX_HD size_t elems() const {
         return period == 1 ? (size_t)side * side : (size_t)period * period * g * g;

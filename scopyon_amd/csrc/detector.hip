@@ -20,8 +20,6 @@
 
 namespace {
 
-constexpr double kTwoPi = 6.283185307179586;
-
 __device__ inline double adc_counts(double x, const smx_detector_params_t& P, double gain,
                                     double offset) {
     // _epifm.py:1475-1484, same order, no rounding
@@ -144,82 +142,72 @@ __global__ void __launch_bounds__(256) emccd_parity_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Philox samplers
+// Philox samplers (float32 arithmetic: these paths are statistical, not
+// bit-parity paths, and float32 transcendental rates are twice float64's)
 // ---------------------------------------------------------------------------
-__device__ inline double normal01(smx::PhiloxStream& st) {
-    double u1 = st.uniform_open0();
-    double u2 = st.uniform();
-    double s, c;
-    sincos(kTwoPi * u2, &s, &c);
-    return sqrt(-2.0 * log(u1)) * c;
-}
+__device__ inline float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }   // [0, 1)
+__device__ inline float u01f_open0(uint32_t w) { return 1.0f - u01f(w); }          // (0, 1]
 
-// Poisson: inversion below 12, Hormann's PTRS (transformed rejection with
-// squeeze, 1993) above.
-__device__ __noinline__ long long poisson_ptrs(smx::PhiloxStream& st, double lam);
-
-__device__ inline long long poisson(smx::PhiloxStream& st, double lam) {
-    if (!(lam > 0.0)) return 0;
-    if (lam >= 12.0) return poisson_ptrs(st, lam);
-    {
-        double u = st.uniform();
-        double p = exp(-lam), s = p;
-        long long k = 0;
-        while (u > s && k < 4096) {
-            ++k;
-            p *= lam / (double)k;
-            s += p;
-        }
-        return k;
+struct FStream {
+    smx::PhiloxStream st;
+    __device__ FStream(uint64_t seed, uint32_t id, uint32_t seq, uint32_t tag, uint32_t start)
+        : st(seed, id, seq, tag) { st.n = start; }
+    __device__ float uniform() { return u01f(st.next32()); }
+    __device__ float uniform_open0() { return u01f_open0(st.next32()); }
+    __device__ float normal() {
+        float r = sqrtf(-2.0f * logf(uniform_open0()));
+        float s, c;
+        sincospif(2.0f * uniform(), &s, &c);
+        return r * c;
     }
-}
+};
 
-__device__ __noinline__ long long poisson_ptrs(smx::PhiloxStream& st, double lam) {
-    const double slam = sqrt(lam), loglam = log(lam);
-    const double b = 0.931 + 2.53 * slam;
-    const double aa = -0.059 + 0.02483 * b;
-    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
+// Poisson for lam >= 12: Hormann's PTRS (transformed rejection with squeeze,
+// Insurance: Mathematics and Economics 12, 1993).
+__device__ __noinline__ long long poisson_ptrs(FStream& st, float lam) {
+    const float slam = sqrtf(lam), loglam = logf(lam);
+    const float b = 0.931f + 2.53f * slam;
+    const float aa = -0.059f + 0.02483f * b;
+    const float invalpha = 1.1239f + 1.1328f / (b - 3.4f);
+    const float vr = 0.9277f - 3.6224f / (b - 2.0f);
     for (int it = 0; it < 1000; ++it) {
-        double U = st.uniform() - 0.5;
-        double V = st.uniform();
-        double us = 0.5 - fabs(U);
-        long long k = (long long)floor((2.0 * aa / us + b) * U + lam + 0.43);
-        if (us >= 0.07 && V <= vr) return k;
-        if (k < 0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + log(invalpha) - log(aa / (us * us) + b) <=
-            -lam + (double)k * loglam - lgamma((double)k + 1.0))
-            return k;
+        float U = st.uniform() - 0.5f;
+        float V = st.uniform();
+        float us = 0.5f - fabsf(U);
+        float kf = floorf((2.0f * aa / us + b) * U + lam + 0.43f);
+        if (us >= 0.07f && V <= vr) return (long long)kf;
+        if (kf < 0.0f || (us < 0.013f && V > us)) continue;
+        if (logf(V) + logf(invalpha) - logf(aa / (us * us) + b) <=
+            -lam + kf * loglam - lgammaf(kf + 1.0f))
+            return (long long)kf;
     }
-    return (long long)floor(lam + 0.5);
+    return (long long)floorf(lam + 0.5f);
 }
 
 // Gamma(shape n >= 1, scale 1): exponential for n == 1, Marsaglia-Tsang
 // (ACM TOMS 26(3), 2000) otherwise.
-__device__ __noinline__ double gamma_int(smx::PhiloxStream& st, long long n) {
-    if (n == 1) return -log(st.uniform_open0());
-    const double d = (double)n - 1.0 / 3.0;
-    const double c = 1.0 / sqrt(9.0 * d);
+__device__ __noinline__ float gamma_int(FStream& st, long long n) {
+    if (n == 1) return -logf(st.uniform_open0());
+    const float d = (float)n - 1.0f / 3.0f;
+    const float c = 1.0f / sqrtf(9.0f * d);
     for (int it = 0; it < 1000; ++it) {
-        double x = normal01(st);
-        double v = 1.0 + c * x;
-        if (v <= 0.0) continue;
+        float x = st.normal();
+        float v = 1.0f + c * x;
+        if (v <= 0.0f) continue;
         v = v * v * v;
-        double u = st.uniform_open0();
-        if (u < 1.0 - 0.0331 * (x * x) * (x * x)) return d * v;
-        if (log(u) < 0.5 * x * x + d * (1.0 - v + log(v))) return d * v;
+        float u = st.uniform_open0();
+        if (u < 1.0f - 0.0331f * (x * x) * (x * x)) return d * v;
+        if (logf(u) < 0.5f * x * x + d * (1.0f - v + logf(v))) return d * v;
     }
     return d;
 }
 
-__device__ inline double emccd_philox_signal(smx::PhiloxStream& st, double E, int g) {
-    if (!(E > 0.0)) return 0.0;
-    long long n = poisson(st, E);
-    if (n == 0) return 0.0;
+// EM-multiplied signal for n >= 1 photoelectrons: round(Gamma(n, g)), >= 1.
+__device__ __noinline__ double emccd_gain_signal(FStream& st, long long n, int g) {
     double s;
     int guard = 0;
     do {
-        s = gamma_int(st, n) * (double)g;
+        s = (double)gamma_int(st, n) * (double)g;
     } while (s < 0.5 && ++guard < 64);
     double S = floor(s + 0.5);
     return S < 1.0 ? 1.0 : S;
@@ -230,15 +218,43 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
                                     const double* __restrict__ gain_map,
                                     const double* __restrict__ offset_map, smx_detector_params_t P,
                                     uint64_t seed, uint32_t frame, uint32_t pix_base,
+                                    const uint32_t* __restrict__ frame_dev,
                                     double* __restrict__ E_out, OutT* __restrict__ adc_out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double E = expectation(photons[i], P);
-    smx::PhiloxStream st(seed, pix_base + (uint32_t)i, frame, 0x0E3CCD01u);
-    double noise = P.readout_noise > 0.0 ? P.readout_noise * normal01(st) : 0.0;
-    double S = emccd_philox_signal(st, E, P.emgain);
-    double gain = gain_map ? gain_map[i] : P.gain;
-    double offset = offset_map ? offset_map[i] : P.offset;
+    if (frame_dev) frame += *frame_dev;
+    const double E = expectation(photons[i], P);
+    const uint32_t id = pix_base + (uint32_t)i;
+    // one Philox block covers the common path: readout noise + Poisson draw
+    smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id, frame, 0x0E3CCD01u, 0u},
+                                      (uint32_t)seed, (uint32_t)(seed >> 32));
+    double noise = 0.0;
+    if (P.readout_noise > 0.0) {
+        float rad = sqrtf(-2.0f * logf(u01f_open0(r.x)));
+        float sn, cs;
+        sincospif(2.0f * u01f(r.y), &sn, &cs);
+        noise = P.readout_noise * (double)(rad * cs);
+    }
+    double S = 0.0;
+    if (E > 0.0) {
+        FStream st(seed, id, frame, 0x0E3CCD01u, 1u);
+        long long k = 0;
+        const float lam = (float)E;
+        if (lam < 12.0f) {
+            float u = u01f(r.z);
+            float p = expf(-lam), cum = p;
+            while (u > cum && k < 4096) {
+                ++k;
+                p *= lam / (float)k;
+                cum += p;
+            }
+        } else {
+            k = poisson_ptrs(st, lam);
+        }
+        if (k > 0) S = emccd_gain_signal(st, k, P.emgain);
+    }
+    const double gain = gain_map ? gain_map[i] : P.gain;
+    const double offset = offset_map ? offset_map[i] : P.offset;
     if (E_out) E_out[i] = E;
     adc_out[i] = (OutT)adc_counts(S + noise, P, gain, offset);
 }
@@ -314,8 +330,9 @@ int smx_emccd_parity(int n, const double* E, const double* noise, const double* 
 
 int smx_emccd_philox(int n, const double* photons, const double* gain_map,
                      const double* offset_map, const smx_detector_params_t* params,
-                     uint64_t seed, uint32_t frame, uint32_t pix_base, double* E_out,
-                     void* adc_out, int adc_is_float32, hipStream_t stream) {
+                     uint64_t seed, uint32_t frame, uint32_t pix_base,
+                     const uint32_t* frame_dev, double* E_out, void* adc_out,
+                     int adc_is_float32, hipStream_t stream) {
     if (n < 0 || !params || params->emgain <= 0 || (n > 0 && (!photons || !adc_out))) {
         smx::set_error("smx_emccd_philox: invalid arguments (n=%d)", n);
         return SMX_EVALUE;
@@ -323,12 +340,12 @@ int smx_emccd_philox(int n, const double* photons, const double* gain_map,
     if (n == 0) return SMX_OK;
     if (adc_is_float32)
         hipLaunchKernelGGL(emccd_philox_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, stream,
-                           n, photons, gain_map, offset_map, *params, seed, frame, pix_base, E_out,
-                           (float*)adc_out);
+                           n, photons, gain_map, offset_map, *params, seed, frame, pix_base,
+                           frame_dev, E_out, (float*)adc_out);
     else
         hipLaunchKernelGGL(emccd_philox_kernel<double>, dim3((n + 255) / 256), dim3(256), 0,
                            stream, n, photons, gain_map, offset_map, *params, seed, frame,
-                           pix_base, E_out, (double*)adc_out);
+                           pix_base, frame_dev, E_out, (double*)adc_out);
     SMX_CHECK_LAUNCH("emccd_philox_kernel");
     return SMX_OK;
 }

@@ -23,9 +23,11 @@
 
 namespace {
 
-constexpr int TS = 16;         // tile side in pixels
+constexpr int TS = 32;         // tile side in pixels
 constexpr int TB = TS + 1;     // boundaries per tile axis
-constexpr int CHUNK = 8;       // emitters staged in LDS per round
+constexpr int PY = 4;          // pixel rows per thread (256 threads x 4 = 32 x 32)
+constexpr int RS = TS / PY;    // row stride between a thread's pixels
+constexpr int CHUNK = 4;       // emitters staged in LDS per round (general path)
 constexpr int SORT_CAP = 4096; // max tile list sorted in LDS
 
 // One axis of the footprint: pixel range [k0, k1) and its cell boundaries.
@@ -175,7 +177,7 @@ struct TileArgs {
     int nfov;
     const int* tile_fov;     // FOV of each global tile
     const int* tile_begin;   // n_tiles + 1 offsets into list
-    const int* list;
+    int* list;               // sorted in place per tile
     long long capacity;      // entries in list
     const double* sat;
     size_t sat_stride;
@@ -208,12 +210,21 @@ __device__ void lds_bitonic_sort(int* a, int n_pow2) {
 }
 
 __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
-    __shared__ int s_list[SORT_CAP];
-    __shared__ int s_rowB[CHUNK][TB];
-    __shared__ int s_colB[CHUNK][TB];
-    __shared__ double s_corner[CHUNK][TB][TB];
+    // LDS: the sort buffer is only live before the main loop; the corner
+    // staging area reuses it afterwards.
+    __shared__ union {
+        int list[SORT_CAP];
+        double corner[CHUNK][TB][TB];
+    } u;
+    __shared__ int s_id[CHUNK];
+    __shared__ int s_k0[CHUNK][2], s_k1[CHUNK][2];
+    __shared__ double s_o[CHUNK][2];
+    __shared__ int s_B[CHUNK][2][TB];          // cell boundary per tile edge (-1: outside)
+    __shared__ long long s_P[CHUNK][2][TB];    // SAT index part of that boundary
     __shared__ double s_w[CHUNK];
     __shared__ long long s_sat_off[CHUNK];
+    __shared__ int s_lo[CHUNK][2], s_n[CHUNK][2];   // valid tile-edge range per axis
+    __shared__ int s_off[CHUNK + 1];                 // prefix of valid corners per emitter
 
     const int tile = blockIdx.x;
     const int fov = A.tile_fov[tile];
@@ -221,45 +232,49 @@ __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
     const smx_fov_t F = A.fovs[fov];
     const int local = tile - F.tile_off;
     const int tr = local / F.tiles_c, tc = local % F.tiles_c;
-    const int ty = threadIdx.x / TS, tx = threadIdx.x % TS;
-    const int pi = tr * TS + ty, pj = tc * TS + tx;   // this thread's pixel (row, col)
-    const bool inside = pi < F.nw && pj < F.nh;
+    const int ty = threadIdx.x / TS, tx = threadIdx.x % TS;   // rows ty + RS*q, column tx
+    const int pj = tc * TS + tx;
 
     const int begin = A.tile_begin[tile];
     int count = A.tile_begin[tile + 1] - begin;
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
-    const int* list = A.list + begin;
+    int* list = A.list + begin;
 
-    // sort the tile list by emitter index (emitter order per pixel)
-    bool staged = count <= SORT_CAP;
-    if (staged && count > 0) {
+    // sort the tile list by emitter index (emitter order per pixel) and write
+    // it back in place
+    if (count > 1 && count <= SORT_CAP) {
         int np2 = 1;
         while (np2 < count) np2 <<= 1;
-        for (int k = threadIdx.x; k < np2; k += blockDim.x) s_list[k] = k < count ? list[k] : 0x7fffffff;
+        for (int k = threadIdx.x; k < np2; k += blockDim.x) u.list[k] = k < count ? list[k] : 0x7fffffff;
         __syncthreads();
-        lds_bitonic_sort(s_list, np2);
-    } else if (!staged && threadIdx.x == 0) {
+        lds_bitonic_sort(u.list, np2);
+        for (int k = threadIdx.x; k < count; k += blockDim.x) list[k] = u.list[k];
+        __syncthreads();
+    } else if (count > SORT_CAP && threadIdx.x == 0) {
         atomicOr(A.status, 1);
     }
 
-    double acc = 0.0;
+    double acc[PY];
+#pragma unroll
+    for (int q = 0; q < PY; ++q) acc[q] = 0.0;
     for (int base = 0; base < count; base += CHUNK) {
         const int nc = min(CHUNK, count - base);
-        // phase A: row/column cell boundaries of this tile for nc emitters
-        for (int t = threadIdx.x; t < nc * 2 * TB; t += blockDim.x) {
-            int c = t / (2 * TB), m = t % (2 * TB);
-            int e = staged ? s_list[base + c] : list[base + c];
-            bool is_row = m < TB;
-            int mm = is_row ? m : m - TB;
-            double xi = is_row ? A.u[e] - F.p0u : A.v[e] - F.p0v;
-            double half = is_row ? F.half_w : F.half_h;
-            int npix = is_row ? F.nw : F.nh;
-            Axis ax = make_axis(half, xi, A.ext_half, F.pixel_length, npix);
-            long long k = (long long)(is_row ? tr : tc) * TS + mm;
-            int b = -1;
-            if (k >= ax.k0 && k <= ax.k1) b = axis_boundary(ax, k, F.pixel_length, A.res, A.ntab);
-            if (is_row) s_rowB[c][mm] = b; else s_colB[c][mm] = b;
-            if (m == 0) {
+        // phase A0: per emitter and axis, the footprint range and cell origin
+        if (threadIdx.x < nc * 2) {
+            int c = threadIdx.x >> 1, ax = threadIdx.x & 1;
+            int e = list[base + c];
+            double xi = ax == 0 ? A.u[e] - F.p0u : A.v[e] - F.p0v;
+            Axis a = make_axis(ax == 0 ? F.half_w : F.half_h, xi, A.ext_half, F.pixel_length,
+                               ax == 0 ? F.nw : F.nh);
+            s_o[c][ax] = a.o;
+            s_k0[c][ax] = (int)a.k0;
+            s_k1[c][ax] = (int)a.k1;
+            long long e0 = (long long)(ax == 0 ? tr : tc) * TS;   // first edge of the tile
+            long long lo = max(a.k0 - e0, 0LL), hi = min(a.k1 - e0, (long long)TS);
+            s_lo[c][ax] = (int)lo;
+            s_n[c][ax] = hi >= lo ? (int)(hi - lo + 1) : 0;
+            if (ax == 0) {
+                s_id[c] = e;
                 int slot = A.key[e];
                 bool ok = slot >= 0 && slot < A.n_slots;
                 s_w[c] = ok ? A.w[e] : 0.0;
@@ -268,37 +283,360 @@ __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
             }
         }
         __syncthreads();
-        // phase B: gather the SAT corners this tile needs
-        for (int t = threadIdx.x; t < nc * TB * TB; t += blockDim.x) {
-            int c = t / (TB * TB), r = t % (TB * TB);
-            int m = r / TB, l = r % TB;
-            int b = s_rowB[c][m], cc = s_colB[c][l];
-            double val = 0.0;
-            if (b >= 0 && cc >= 0 && s_sat_off[c] >= 0) val = A.sat[s_sat_off[c] + A.L.index(b, cc)];
-            s_corner[c][m][l] = val;
+        // phase A1: cell boundaries of this tile's edges inside each footprint
+        for (int t = threadIdx.x; t < nc * 2 * TB; t += blockDim.x) {
+            int c = t / (2 * TB), r = t % (2 * TB);
+            int ax = r / TB, m = r % TB;
+            long long k = (long long)(ax == 0 ? tr : tc) * TS + m;
+            int b = -1;
+            long long part = 0;
+            if (k >= s_k0[c][ax] && k <= s_k1[c][ax]) {
+                Axis a;
+                a.o = s_o[c][ax];
+                a.k0 = s_k0[c][ax];
+                a.k1 = s_k1[c][ax];
+                b = axis_boundary(a, k, F.pixel_length, A.res, A.ntab);
+                part = (long long)(ax == 0 ? A.L.row_part(b) : A.L.col_part(b));
+            }
+            s_B[c][ax][m] = b;
+            s_P[c][ax][m] = part;
+        }
+        __syncthreads();
+        // phase B: gather only the valid SAT corners (compacted over the chunk),
+        // four independent loads in flight per thread
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int c = 0; c < nc; ++c) {
+                s_off[c] = run;
+                run += s_sat_off[c] >= 0 ? s_n[c][0] * s_n[c][1] : 0;
+            }
+            s_off[nc] = run;
+        }
+        __syncthreads();
+        {
+            const int total = s_off[nc];
+            for (int t0 = threadIdx.x; t0 < total; t0 += 4 * 256) {
+                double vals[4];
+                int dst[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int t = t0 + q * 256;
+                    dst[q] = -1;
+                    vals[q] = 0.0;
+                    if (t < total) {
+                        int c = 0;
+                        while (c + 1 < nc && s_off[c + 1] <= t) ++c;
+                        int r = t - s_off[c];
+                        int m = s_lo[c][0] + r / s_n[c][1];
+                        int l = s_lo[c][1] + r % s_n[c][1];
+                        vals[q] = smx::gload(A.sat + s_sat_off[c] + s_P[c][0][m] + s_P[c][1][l]);
+                        dst[q] = (c * TB + m) * TB + l;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (dst[q] >= 0) (&u.corner[0][0][0])[dst[q]] = vals[q];
+            }
         }
         __syncthreads();
         // phase C: per-pixel accumulation in emitter order
-        if (inside) {
+        if (pj < F.nh) {
             for (int c = 0; c < nc; ++c) {
-                int b0 = s_rowB[c][ty], b1 = s_rowB[c][ty + 1];
-                int c0 = s_colB[c][tx], c1 = s_colB[c][tx + 1];
-                if (b0 < 0 || b1 < 0 || c0 < 0 || c1 < 0 || s_sat_off[c] < 0) continue;
-                if (b1 <= b0 || c1 <= c0) continue;
-                double s = ((s_corner[c][ty + 1][tx + 1] - s_corner[c][ty][tx + 1])
-                            - s_corner[c][ty + 1][tx]) + s_corner[c][ty][tx];
-                double photons = s * A.unit_area;
-                if (photons > 0.0) acc += photons * s_w[c];
+                if (s_sat_off[c] < 0) continue;
+                int c0 = s_B[c][1][tx], c1 = s_B[c][1][tx + 1];
+                if (c0 < 0 || c1 < 0 || c1 <= c0) continue;
+#pragma unroll
+                for (int q = 0; q < PY; ++q) {
+                    int r = ty + RS * q;
+                    int b0 = s_B[c][0][r], b1 = s_B[c][0][r + 1];
+                    if (b0 < 0 || b1 < 0 || b1 <= b0) continue;
+                    double sv = ((u.corner[c][r + 1][tx + 1] - u.corner[c][r][tx + 1])
+                                 - u.corner[c][r + 1][tx]) + u.corner[c][r][tx];
+                    double photons = sv * A.unit_area;
+                    if (photons > 0.0) acc[q] += photons * s_w[c];
+                }
             }
         }
         __syncthreads();
     }
-    if (inside) {
-        size_t o = (size_t)F.pix_off + (size_t)pi * F.nh + pj;
-        if (A.accumulate) {
-            if (count > 0) A.out[o] = A.out[o] + acc;
-        } else {
-            A.out[o] = acc;
+    if (pj < F.nh) {
+#pragma unroll
+        for (int q = 0; q < PY; ++q) {
+            int pi = tr * TS + ty + RS * q;
+            if (pi >= F.nw) continue;
+            size_t o = (size_t)F.pix_off + (size_t)pi * F.nh + pj;
+            if (A.accumulate) {
+                if (count > 0) A.out[o] = A.out[o] + acc[q];
+            } else {
+                A.out[o] = acc[q];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Phase-table fast path (pixel length = P table cells, P integer)
+//
+// With pl = P * res every emitter's interior cell boundaries along an axis
+// are phi + P * (m - 1) for m >= 1 (phi in [1, P]) and 0 for m = 0, clamped
+// to the table end -- the reference's ceil((k pl - o)/res) with its two
+// clamps.  The pixel integrals therefore depend only on (phi_u, phi_v) and
+// the pixel's offset (du, dv) inside the footprint, so they are tabulated
+// once per depth key: K[phi_u][phi_v][du][dv] = the same four-corner SAT
+// difference the general path evaluates (bit-identical values).  The splat
+// then costs one coalesced load per (emitter, pixel).  An emitter whose
+// boundaries are not on the regular grid (a float64 rounding tie in
+// ceil((k pl - o)/res), probability ~1e-12 per boundary) is flagged and
+// integrated through the SAT instead.
+// --------------------------------------------------------------------------
+struct EmitterRec {
+    int lo_u, lo_v;          // first footprint pixel per axis (may be < 0)
+    short n_u, n_v;          // footprint pixels per axis (hi - lo)
+    int flags;               // 1: irregular boundaries -> SAT path; 2: no table slot
+    double w;
+    long long table_off;     // offset of K[phi_u][phi_v][0][0]
+};
+
+__device__ inline int regular_phase(const Axis& a, double half, double xi, double ext_half,
+                                    double pl, double res, int ntab, int P, int M,
+                                    long long lo, long long hi, bool& regular) {
+    // phi = ceil(((lo + 1) pl - o) / res); check every boundary lo+1..hi
+    long long phi = smx::ceil_i64(((double)(lo + 1) * pl - a.o) / res);
+    if (phi < 1 || phi > P || hi - lo > M) regular = false;
+    for (long long k = lo + 1; regular && k <= hi; ++k) {
+        long long b = smx::ceil_i64(((double)k * pl - a.o) / res);
+        long long want = phi + (long long)P * (k - lo - 1);
+        if (k == hi) {
+            b = b > ntab ? ntab : b;
+            want = want > ntab ? ntab : want;
+        }
+        if (b != want) regular = false;
+    }
+    return (int)(phi - 1);
+}
+
+__global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const double* __restrict__ v,
+                                   const double* __restrict__ w, const int* __restrict__ key,
+                                   const int* __restrict__ fov_id, const smx_fov_t* __restrict__ fovs,
+                                   int nfov, double ext_half, double res, int ntab, int P, int M,
+                                   int n_slots, size_t k_stride, EmitterRec* __restrict__ rec) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    EmitterRec r;
+    r.lo_u = r.lo_v = 0;
+    r.n_u = r.n_v = 0;
+    r.flags = 0;
+    r.w = w[i];
+    r.table_off = 0;
+    int f = fov_of(fov_id, i);
+    int slot = key[i];
+    if (!fov_ok(f, nfov) || slot < 0 || slot >= n_slots) {
+        r.flags = 2;
+        rec[i] = r;
+        return;
+    }
+    const smx_fov_t F = fovs[f];
+    const double xi = u[i] - F.p0u, yi = v[i] - F.p0v;
+    bool regular = true;
+    int ph[2];
+    for (int ax = 0; ax < 2; ++ax) {
+        double half = ax == 0 ? F.half_w : F.half_h;
+        double x = ax == 0 ? xi : yi;
+        double base = half + x;
+        Axis a;
+        a.o = base - ext_half;
+        long long lo = (long long)floor(a.o / F.pixel_length);
+        long long hi = (long long)ceil((base + ext_half) / F.pixel_length);
+        ph[ax] = regular_phase(a, half, x, ext_half, F.pixel_length, res, ntab, P, M, lo, hi,
+                               regular);
+        if (lo < -(1LL << 30) || lo > (1LL << 30)) regular = false;
+        long long span = hi - lo;
+        short len = (short)(span < 0 ? 0 : (span > 32767 ? 32767 : span));
+        if (ax == 0) { r.lo_u = (int)lo; r.n_u = len; } else { r.lo_v = (int)lo; r.n_v = len; }
+    }
+    if (!regular) {
+        r.flags = 1;
+    } else {
+        r.table_off = (long long)slot * (long long)k_stride +
+                      ((long long)ph[0] * P + ph[1]) * (long long)(M * M);
+    }
+    rec[i] = r;
+}
+
+// K[slot][phi_u][phi_v][du][dv] from the SATs (same arithmetic as the tile kernel).
+__global__ void phase_table_kernel(const double* __restrict__ sat, size_t sat_stride,
+                                   smx::SatLayout L, int ntab, int P, int M, int nslots,
+                                   double unit_area, double* __restrict__ K) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t per = (size_t)P * P * M * M;
+    if (idx >= per * nslots) return;
+    size_t slot = idx / per, r = idx % per;
+    int dv = (int)(r % M); r /= M;
+    int du = (int)(r % M); r /= M;
+    int pv = (int)(r % P), pu = (int)(r / P);
+    auto edge = [&](int phi, int m) -> int {
+        if (m == 0) return 0;
+        long long b = (long long)phi + (long long)P * (m - 1);
+        return (int)(b > ntab ? ntab : b);
+    };
+    int b0 = edge(pu + 1, du), b1 = edge(pu + 1, du + 1);
+    int c0 = edge(pv + 1, dv), c1 = edge(pv + 1, dv + 1);
+    double val = 0.0;
+    if (b1 > b0 && c1 > c0) {
+        const double* S = sat + slot * sat_stride;
+        val = ((S[L.index(b1, c1)] - S[L.index(b0, c1)]) - S[L.index(b1, c0)]) + S[L.index(b0, c0)];
+    }
+    // photons = sum * unit_area, kept only when positive (the reference's test)
+    double photons = val * unit_area;
+    K[idx] = photons > 0.0 ? photons : 0.0;
+}
+
+struct FastArgs {
+    const double* u;
+    const double* v;
+    const int* key;
+    const smx_fov_t* fovs;
+    int nfov;
+    const int* tile_fov;
+    const int* tile_begin;
+    int* list;
+    long long capacity;
+    const EmitterRec* rec;
+    const double* K;
+    int M;
+    const double* sat;
+    size_t sat_stride;
+    smx::SatLayout L;
+    double res, ext_half, unit_area;
+    int ntab;
+    int accumulate;
+    double* out;
+    int* status;
+};
+
+// General-path pixel integral for one emitter (used for flagged emitters).
+__device__ __noinline__ double sat_pixel(const struct FastArgs& A, const smx_fov_t& F, int e,
+                                         double w, int pi, int pj, double acc);
+
+constexpr int FCHUNK = 64;
+
+__device__ __noinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& F, int e, double w,
+                                         int pi, int pj, double acc) {
+    int slot = A.key[e];
+    Axis au = make_axis(F.half_w, A.u[e] - F.p0u, A.ext_half, F.pixel_length, F.nw);
+    Axis av = make_axis(F.half_h, A.v[e] - F.p0v, A.ext_half, F.pixel_length, F.nh);
+    if (pi < au.k0 || pi >= au.k1 || pj < av.k0 || pj >= av.k1) return acc;
+    int b0 = axis_boundary(au, pi, F.pixel_length, A.res, A.ntab);
+    int b1 = axis_boundary(au, pi + 1, F.pixel_length, A.res, A.ntab);
+    int c0 = axis_boundary(av, pj, F.pixel_length, A.res, A.ntab);
+    int c1 = axis_boundary(av, pj + 1, F.pixel_length, A.res, A.ntab);
+    if (b1 <= b0 || c1 <= c0) return acc;
+    const double* S = A.sat + (size_t)slot * A.sat_stride;
+    double s = ((S[A.L.index(b1, c1)] - S[A.L.index(b0, c1)]) - S[A.L.index(b1, c0)]) +
+               S[A.L.index(b0, c0)];
+    double photons = s * A.unit_area;
+    if (photons > 0.0) acc += photons * w;
+    return acc;
+}
+
+// Each wave owns one 8-pixel-wide column strip of the 32 x 32 tile (four 8 x 8
+// blocks, q = 0..3); an emitter is skipped block by block with wave-uniform
+// tests, so only blocks its footprint touches cost anything.
+__global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
+    __shared__ int s_sort[SORT_CAP];
+    __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (tile-relative), n_u, n_v
+    __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
+    __shared__ double s_w[FCHUNK];
+    __shared__ const double* s_kb[FCHUNK];     // K block of the emitter's phase
+
+    const int tile = blockIdx.x;
+    const int fov = A.tile_fov[tile];
+    if (fov < 0 || fov >= A.nfov) return;
+    const smx_fov_t F = A.fovs[fov];
+    const int local = tile - F.tile_off;
+    const int tr = local / F.tiles_c, tc = local % F.tiles_c;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ly = lane >> 3, lx = lane & 7;        // position inside an 8 x 8 block
+    const int cx = wave * 8 + lx;                   // tile-relative column
+    const int pj = tc * TS + cx;
+
+    const int begin = A.tile_begin[tile];
+    int count = A.tile_begin[tile + 1] - begin;
+    if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
+    const int* list = A.list + begin;
+    const bool sorted = count <= SORT_CAP;
+    if (count >= 1 && sorted) {
+        int np2 = 1;
+        while (np2 < count) np2 <<= 1;
+        for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
+        __syncthreads();
+        if (count > 1) lds_bitonic_sort(s_sort, np2);
+    } else if (!sorted && threadIdx.x == 0) {
+        atomicOr(A.status, 1);
+    }
+
+    const int M = A.M;
+    double acc[PY];
+#pragma unroll
+    for (int q = 0; q < PY; ++q) acc[q] = 0.0;
+    for (int base = 0; base < count; base += FCHUNK) {
+        const int nc = min(FCHUNK, count - base);
+        if (threadIdx.x < nc) {
+            int e = sorted ? s_sort[base + threadIdx.x] : smx::gload(list + base + threadIdx.x);
+            EmitterRec r = A.rec[e];
+            s_e[threadIdx.x] = e;
+            s_geo[threadIdx.x] = make_int4(r.lo_u - tr * TS, r.lo_v - tc * TS, r.n_u, r.n_v);
+            s_flags[threadIdx.x] = r.flags;
+            s_w[threadIdx.x] = r.w;
+            s_kb[threadIdx.x] = A.K + (r.flags == 0 ? r.table_off : 0);
+            if (r.flags & 2) atomicOr(A.status, 8);
+        }
+        __syncthreads();
+        if (pj < F.nh) {
+            for (int c = 0; c < nc; ++c) {
+                const int4 g = s_geo[c];
+                const int flags = s_flags[c];
+                if (flags & 2) continue;
+                // this wave's columns [8 wave, 8 wave + 8) against the footprint
+                if (g.y >= wave * 8 + 8 || g.y + g.w <= wave * 8) continue;
+                if (flags == 0) {
+                    const double* kb = s_kb[c];
+                    const double w = s_w[c];
+                    const int dv = cx - g.y;
+                    const bool col_ok = (unsigned)dv < (unsigned)g.w;
+                    double val[PY];
+#pragma unroll
+                    for (int q = 0; q < PY; ++q) {
+                        int du = q * 8 + ly - g.x;
+                        bool hit = col_ok && (unsigned)du < (unsigned)g.z;
+                        val[q] = smx::gload(kb + (hit ? du * M + dv : 0));
+                        val[q] = hit ? val[q] : 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < PY; ++q)
+                        if (g.x < q * 8 + 8 && g.x + g.z > q * 8) acc[q] += val[q] * w;
+                } else if (flags == 1) {
+#pragma unroll
+                    for (int q = 0; q < PY; ++q) {
+                        int pi = tr * TS + q * 8 + ly;
+                        if (pi < F.nw) acc[q] = sat_pixel(A, F, s_e[c], s_w[c], pi, pj, acc[q]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (pj < F.nh) {
+#pragma unroll
+        for (int q = 0; q < PY; ++q) {
+            int pi = tr * TS + q * 8 + ly;
+            if (pi >= F.nw) continue;
+            size_t o = (size_t)F.pix_off + (size_t)pi * F.nh + pj;
+            if (A.accumulate) {
+                if (count > 0) A.out[o] = A.out[o] + acc[q];
+            } else {
+                A.out[o] = acc[q];
+            }
         }
     }
 }
@@ -316,6 +654,7 @@ __global__ void tile_fov_kernel(int nfov, const smx_fov_t* __restrict__ fovs, in
 }
 
 struct SplatWorkspace {
+    EmitterRec* rec;   // n (phase-table path)
     int* tile_count;   // n_tiles + 1
     int* tile_begin;   // n_tiles + 1
     int* tile_cursor;  // n_tiles
@@ -333,9 +672,10 @@ size_t scan_tmp_bytes(int n_tiles) {
     return bytes;
 }
 
-size_t layout_workspace(int n_tiles, long long capacity, char* base, SplatWorkspace* ws) {
+size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, SplatWorkspace* ws) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    size_t o_rec = take(sizeof(EmitterRec) * (size_t)(n > 0 ? n : 1));
     size_t o_count = take(sizeof(int) * (n_tiles + 1));
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
     size_t o_cursor = take(sizeof(int) * n_tiles);
@@ -344,6 +684,7 @@ size_t layout_workspace(int n_tiles, long long capacity, char* base, SplatWorksp
     size_t cb = scan_tmp_bytes(n_tiles);
     size_t o_cub = take(cb);
     if (ws && base) {
+        ws->rec = (EmitterRec*)(base + o_rec);
         ws->tile_count = (int*)(base + o_count);
         ws->tile_begin = (int*)(base + o_begin);
         ws->tile_cursor = (int*)(base + o_cursor);
@@ -383,31 +724,52 @@ int smx_splat_tiles(const smx_fov_t* fov, int tile) {
 
 int smx_splat_tile_size(void) { return TS; }
 
-size_t smx_splat_workspace_bytes(int n_tiles, long long list_capacity) {
-    if (n_tiles <= 0 || list_capacity < 0) return 0;
-    return layout_workspace(n_tiles, list_capacity, nullptr, nullptr);
+size_t smx_splat_workspace_bytes(int n_emitters, int n_tiles, long long list_capacity) {
+    if (n_emitters < 0 || n_tiles <= 0 || list_capacity < 0) return 0;
+    return layout_workspace(n_emitters, n_tiles, list_capacity, nullptr, nullptr);
+}
+
+size_t smx_phase_table_elems(int period, int m) {
+    if (period < 1 || m < 1) return 0;
+    return (size_t)period * period * m * m;
+}
+
+int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int ntab,
+                          int sat_period, int m, double unit_area, double* K, hipStream_t stream) {
+    if (!sat || !K || nslots < 0 || ntab < 1 || sat_period < 2 || m < 1) {
+        smx::set_error("smx_phase_table_build: invalid arguments (period=%d, m=%d)", sat_period, m);
+        return SMX_EVALUE;
+    }
+    if (nslots == 0) return SMX_OK;
+    size_t total = smx_phase_table_elems(sat_period, m) * (size_t)nslots;
+    smx::SatLayout L = smx::make_sat_layout(ntab + 1, sat_period);
+    hipLaunchKernelGGL(phase_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, sat, sat_stride, L, ntab, sat_period, m, nslots, unit_area, K);
+    SMX_CHECK_LAUNCH("phase_table_kernel");
+    return SMX_OK;
 }
 
 int smx_splat_expected(int n, const double* u, const double* v, const double* w, const int* key,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
                        const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                        double res, double unit_area, long long list_capacity, int accumulate,
-                       double* out, void* workspace, size_t workspace_bytes, int* status_dev,
-                       hipStream_t stream) {
+                       const double* K, int m, double* out, void* workspace,
+                       size_t workspace_bytes, int* status_dev, hipStream_t stream) {
     if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev || !out ||
+        (K && (sat_period < 2 || m < 1)) ||
         !workspace || !status_dev || (n > 0 && (!u || !v || !w || !key || !sat))) {
         smx::set_error("smx_splat_expected: invalid arguments (n=%d, nfov=%d, n_tiles=%d)", n,
                        nfov, n_tiles);
         return SMX_EVALUE;
     }
-    size_t need = layout_workspace(n_tiles, list_capacity, nullptr, nullptr);
+    size_t need = layout_workspace(n, n_tiles, list_capacity, nullptr, nullptr);
     if (workspace_bytes < need) {
         smx::set_error("smx_splat_expected: workspace too small (%zu < %zu bytes)",
                        workspace_bytes, need);
         return SMX_EVALUE;
     }
     SplatWorkspace ws;
-    layout_workspace(n_tiles, list_capacity, (char*)workspace, &ws);
+    layout_workspace(n, n_tiles, list_capacity, (char*)workspace, &ws);
     const double ext_half = (res * (double)(ntab - 1)) * 0.5;   // signal_width * 0.5
     const int nb = (n + 255) / 256;
 
@@ -438,6 +800,24 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                            fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin, ws.tile_cursor, ws.list,
                            list_capacity, status_dev);
         SMX_CHECK_LAUNCH("bin_fill_kernel");
+    }
+    if (K) {
+        if (n > 0) {
+            hipLaunchKernelGGL(emitter_rec_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, key,
+                               fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m, n_slots,
+                               smx_phase_table_elems(sat_period, m), ws.rec);
+            SMX_CHECK_LAUNCH("emitter_rec_kernel");
+        }
+        FastArgs B;
+        B.u = u; B.v = v; B.key = key; B.fovs = fovs_dev; B.nfov = nfov;
+        B.tile_fov = ws.tile_fov; B.tile_begin = ws.tile_begin; B.list = ws.list;
+        B.capacity = list_capacity; B.rec = ws.rec; B.K = K; B.M = m;
+        B.sat = sat; B.sat_stride = sat_stride; B.L = smx::make_sat_layout(ntab + 1, sat_period);
+        B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
+        B.accumulate = accumulate; B.out = out; B.status = status_dev;
+        hipLaunchKernelGGL(splat_phase_kernel, dim3(n_tiles), dim3(256), 0, stream, B);
+        SMX_CHECK_LAUNCH("splat_phase_kernel");
+        return SMX_OK;
     }
     TileArgs A;
     A.u = u; A.v = v; A.w = w; A.key = key; A.fov_id = fov_id; A.fovs = fovs_dev; A.nfov = nfov;

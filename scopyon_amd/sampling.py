@@ -171,7 +171,7 @@ def _philox_walk(rng, x0, Dper, moves, boundary=0, lower=None, upper=None):
     for k, dt in enumerate(moves):
         _lib.call('smx_brownian_philox', n, ndim, xs[0].data_ptr(), xs[1].data_ptr(),
                   xs[2].data_ptr(), d_D.data_ptr(), 0.0, float(dt), boundary, lo, hi, rng.seed,
-                  rng.next_counter(), 0, _device.stream())
+                  rng.next_counter(), 0, None, 0, None, None, _device.stream())
         for d in range(ndim):
             traj[k + 1, :, d] = xs[d].cpu().numpy()
     return traj
