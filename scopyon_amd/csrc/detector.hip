@@ -143,74 +143,67 @@ __global__ void __launch_bounds__(256) emccd_parity_kernel(
 
 // ---------------------------------------------------------------------------
 // Philox samplers (float32 arithmetic: these paths are statistical, not
-// bit-parity paths, and float32 transcendental rates are twice float64's)
+// bit-parity paths).  Every draw comes from a fixed Philox block of the
+// pixel's stream -- block 0: readout noise + Poisson uniform + one spare
+// word; blocks 1..: PTRS iterations; blocks 2^16..: Gamma iterations -- so
+// the sampler needs no stream state and stays inlined (no call spills).
 // ---------------------------------------------------------------------------
 __device__ inline float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }   // [0, 1)
 __device__ inline float u01f_open0(uint32_t w) { return 1.0f - u01f(w); }          // (0, 1]
 
-struct FStream {
-    smx::PhiloxStream st;
-    __device__ FStream(uint64_t seed, uint32_t id, uint32_t seq, uint32_t tag, uint32_t start)
-        : st(seed, id, seq, tag) { st.n = start; }
-    __device__ float uniform() { return u01f(st.next32()); }
-    __device__ float uniform_open0() { return u01f_open0(st.next32()); }
-    __device__ float normal() {
-        float r = sqrtf(-2.0f * logf(uniform_open0()));
-        float s, c;
-        sincospif(2.0f * uniform(), &s, &c);
-        return r * c;
+struct PixelRng {
+    uint32_t id, frame, k0, k1;
+    __device__ smx::u32x4 block(uint32_t b) const {
+        return smx::philox4x32_10(smx::u32x4{id, frame, 0x0E3CCD01u, b}, k0, k1);
     }
 };
 
+__device__ inline float normal_f(uint32_t a, uint32_t b) {
+    float r = sqrtf(-2.0f * logf(u01f_open0(a)));
+    float s, c;
+    sincospif(2.0f * u01f(b), &s, &c);
+    return r * c;
+}
+
 // Poisson for lam >= 12: Hormann's PTRS (transformed rejection with squeeze,
 // Insurance: Mathematics and Economics 12, 1993).
-__device__ __noinline__ long long poisson_ptrs(FStream& st, float lam) {
+__device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
     const float slam = sqrtf(lam), loglam = logf(lam);
     const float b = 0.931f + 2.53f * slam;
     const float aa = -0.059f + 0.02483f * b;
     const float invalpha = 1.1239f + 1.1328f / (b - 3.4f);
     const float vr = 0.9277f - 3.6224f / (b - 2.0f);
-    for (int it = 0; it < 1000; ++it) {
-        float U = st.uniform() - 0.5f;
-        float V = st.uniform();
+    for (uint32_t it = 0; it < 512; ++it) {
+        smx::u32x4 r = R.block(1u + (it >> 1));
+        float U = u01f((it & 1) ? r.z : r.x) - 0.5f;
+        float V = u01f((it & 1) ? r.w : r.y);
         float us = 0.5f - fabsf(U);
         float kf = floorf((2.0f * aa / us + b) * U + lam + 0.43f);
-        if (us >= 0.07f && V <= vr) return (long long)kf;
+        if (us >= 0.07f && V <= vr) return (int)kf;
         if (kf < 0.0f || (us < 0.013f && V > us)) continue;
         if (logf(V) + logf(invalpha) - logf(aa / (us * us) + b) <=
             -lam + kf * loglam - lgammaf(kf + 1.0f))
-            return (long long)kf;
+            return (int)kf;
     }
-    return (long long)floorf(lam + 0.5f);
+    return (int)floorf(lam + 0.5f);
 }
 
-// Gamma(shape n >= 1, scale 1): exponential for n == 1, Marsaglia-Tsang
-// (ACM TOMS 26(3), 2000) otherwise.
-__device__ __noinline__ float gamma_int(FStream& st, long long n) {
-    if (n == 1) return -logf(st.uniform_open0());
+// Gamma(n, 1), n >= 2: Marsaglia-Tsang (ACM TOMS 26(3), 2000), one Philox
+// block per iteration (normal from two words, uniform from the third).
+__device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk) {
     const float d = (float)n - 1.0f / 3.0f;
     const float c = 1.0f / sqrtf(9.0f * d);
-    for (int it = 0; it < 1000; ++it) {
-        float x = st.normal();
+    for (int it = 0; it < 256; ++it) {
+        smx::u32x4 r = R.block(blk++);
+        float x = normal_f(r.x, r.y);
         float v = 1.0f + c * x;
         if (v <= 0.0f) continue;
         v = v * v * v;
-        float u = st.uniform_open0();
+        float u = u01f_open0(r.z);
         if (u < 1.0f - 0.0331f * (x * x) * (x * x)) return d * v;
         if (logf(u) < 0.5f * x * x + d * (1.0f - v + logf(v))) return d * v;
     }
     return d;
-}
-
-// EM-multiplied signal for n >= 1 photoelectrons: round(Gamma(n, g)), >= 1.
-__device__ __noinline__ double emccd_gain_signal(FStream& st, long long n, int g) {
-    double s;
-    int guard = 0;
-    do {
-        s = (double)gamma_int(st, n) * (double)g;
-    } while (s < 0.5 && ++guard < 64);
-    double S = floor(s + 0.5);
-    return S < 1.0 ? 1.0 : S;
 }
 
 template <typename OutT>
@@ -224,24 +217,16 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
     if (i >= n) return;
     if (frame_dev) frame += *frame_dev;
     const double E = expectation(photons[i], P);
-    const uint32_t id = pix_base + (uint32_t)i;
-    // one Philox block covers the common path: readout noise + Poisson draw
-    smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id, frame, 0x0E3CCD01u, 0u},
-                                      (uint32_t)seed, (uint32_t)(seed >> 32));
+    PixelRng R{pix_base + (uint32_t)i, frame, (uint32_t)seed, (uint32_t)(seed >> 32)};
+    const smx::u32x4 r0 = R.block(0);
     double noise = 0.0;
-    if (P.readout_noise > 0.0) {
-        float rad = sqrtf(-2.0f * logf(u01f_open0(r.x)));
-        float sn, cs;
-        sincospif(2.0f * u01f(r.y), &sn, &cs);
-        noise = P.readout_noise * (double)(rad * cs);
-    }
+    if (P.readout_noise > 0.0) noise = P.readout_noise * (double)normal_f(r0.x, r0.y);
     double S = 0.0;
     if (E > 0.0) {
-        FStream st(seed, id, frame, 0x0E3CCD01u, 1u);
-        long long k = 0;
         const float lam = (float)E;
+        int k = 0;
         if (lam < 12.0f) {
-            float u = u01f(r.z);
+            float u = u01f(r0.z);
             float p = expf(-lam), cum = p;
             while (u > cum && k < 4096) {
                 ++k;
@@ -249,9 +234,27 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
                 cum += p;
             }
         } else {
-            k = poisson_ptrs(st, lam);
+            k = poisson_ptrs(R, lam);
         }
-        if (k > 0) S = emccd_gain_signal(st, k, P.emgain);
+        if (k > 0) {
+            // EM register: S = round(Gamma(k, g)), redrawn below 0.5 (S >= 1)
+            uint32_t blk = 1u << 16;
+            uint32_t first = r0.w;
+            double s = 0.0;
+            for (int guard = 0; guard < 64; ++guard) {
+                float gam;
+                if (k == 1) {
+                    gam = -logf(u01f_open0(first));
+                } else {
+                    gam = gamma_mt(R, k, blk);
+                }
+                s = (double)gam * (double)P.emgain;
+                if (s >= 0.5) break;
+                first = R.block(blk++).x;
+            }
+            S = floor(s + 0.5);
+            if (S < 1.0) S = 1.0;
+        }
     }
     const double gain = gain_map ? gain_map[i] : P.gain;
     const double offset = offset_map ? offset_map[i] : P.offset;

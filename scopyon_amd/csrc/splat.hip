@@ -615,7 +615,8 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
 #pragma unroll
                     for (int q = 0; q < PY; ++q)
                         if (g.x < q * 8 + 8 && g.x + g.z > q * 8) acc[q] += val[q] * w;
-                } else if (flags == 1) {
+                } else {
+                    // irregular boundaries: reference integration through the SAT
 #pragma unroll
                     for (int q = 0; q < PY; ++q) {
                         int pi = tr * TS + q * 8 + ly;
