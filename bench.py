@@ -67,8 +67,8 @@ def _splat_only_time(sim, reps):
         _lib.call('smx_splat_expected', sim.n, sim.u.data_ptr(), sim.v.data_ptr(),
                   sim.w.data_ptr(), sim.key.data_ptr(), sim.fov_id.data_ptr(),
                   sim.fovs_dev.data_ptr(), sim.nfov, sim.n_tiles, t.sat.data_ptr(), t.sat_elems,
-                  len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, sim.capacity, 1,
-                  _device.ptr(t.K if sim.use_phase else None), t.m, sim.photons.data_ptr(),
+                  len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, sim.capacity, 0,
+                  _device.ptr(sim.Ktab), 1 if sim.k_float else 0, t.m, sim.photons.data_ptr(),
                   sim.ws.data_ptr(), sim.ws.numel(), sim.splatter.status.data_ptr(), st)
     splat()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -88,8 +88,6 @@ def cpu_baseline(size, n_emit, sample_emitters, seed=1):
     import numpy
     import torch
     from oracle import epifm as O, params as P
-    import scopyon_amd
-    from scopyon_amd import _epifm
     ocfg = P.default_config()
     ocfg['magnification'] = 100.0
     ocfg['detector']['image_size'] = [size, size]
@@ -117,12 +115,12 @@ def cpu_baseline(size, n_emit, sample_emitters, seed=1):
     O.adc_counts(sig + noise, p.adc_fullwell, p.adc_gain, p.adc_offset, p.adc_bit)
     t_det = time.perf_counter() - t0
     per_frame = t_splat * (n_emit / sample_emitters) + t_det
-    # GPU noise-free image on the same particles
-    cfg = _config(size, 0.033)
-    sim = _epifm._EPIFMSimulator(_epifm.EPIFMConfigs(cfg.default, rng=numpy.random.RandomState(0)))
-    camera = torch.zeros(size * size, dtype=torch.float64, device=sim.device)
-    sim._substep(parts, 0.033, camera, None, None, {}, False)
-    got = camera.cpu().numpy().reshape(size, size)
+    # the batched path's noise-free image on the same particles
+    from scopyon_amd.batched import BatchedSimulator
+    sim = BatchedSimulator(_config(size, 0.033), [(size, size)], sample_emitters, dt=0.033, seed=0)
+    sim.set_positions(parts[:, 0], parts[:, 1], parts[:, 2])
+    sim.form()
+    got = sim.frame(0, 'photons').cpu().numpy()
     Eg = p.QE * (got + p.background_mean)
     rel = float(numpy.max(numpy.abs(Eg - E) / numpy.abs(E)))
     return {'value': 1.0 / per_frame, 'unit': 'frames/s', 'cores': 1, 'kind': 'port',

@@ -86,6 +86,8 @@ size_t smx_phase_table_elems(int sat_period, int m);
 int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int ntab,
                           int sat_period, int m, double unit_area, double* K,
                           hipStream_t stream);
+/* float32 copy of a phase table (n elements) for the reduced-traffic splat. */
+int smx_phase_table_to_float(const double* K, size_t n, float* K32, hipStream_t stream);
 /* Pixel integration of n emitters into nfov frames: replaces
  * PointSpreadingFunction.overlay_signal_ called per emitter
  * (_epifm.py:223-282 via _epifm.py:1261-1264).  u, v: absolute lateral
@@ -93,7 +95,8 @@ int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int 
  * (NULL: all in FOV 0).  sat: tables of sat_stride doubles each.
  * accumulate=1 adds into `out` (camera += expected); 0 overwrites.
  * K (may be NULL): phase tables from smx_phase_table_build for the same slots
- * (identical results, one load per emitter-pixel instead of four).
+ * (identical results, one load per emitter-pixel instead of four), or their
+ * float32 copy when k_is_float (each contribution rounded to float32).
  * status bit 1: a tile list exceeded the LDS sort capacity (still correct,
  * order not by emitter index); bit 2: list capacity too small (result
  * incomplete); bit 8: an emitter's table slot was not built (skipped). */
@@ -101,7 +104,7 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
                        const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                        double res, double unit_area, long long list_capacity, int accumulate,
-                       const double* K, int m, double* out, void* workspace,
+                       const void* K, int k_is_float, int m, double* out, void* workspace,
                        size_t workspace_bytes, int* status_dev, hipStream_t stream);
 
 /* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */

@@ -280,7 +280,7 @@ class Splatter:
         _lib.call('smx_splat_expected', n, _device.ptr(u), _device.ptr(v), _device.ptr(w),
                   _device.ptr(key), _device.ptr(fov_id), fovs_dev.data_ptr(), nfov, n_tiles,
                   t.sat.data_ptr(), t.sat_elems, len(t.slot_of), t.ntab, t.period, RES, RES * RES,
-                  cap, 1 if accumulate else 0, _device.ptr(K), t.m, out.data_ptr(),
+                  cap, 1 if accumulate else 0, _device.ptr(K), 0, t.m, out.data_ptr(),
                   ws.data_ptr(), ws.numel(), self.status.data_ptr(), _device.stream())
 
 

@@ -69,6 +69,7 @@ class PSFTables:
         self.k_elems = (int(_lib.lib().smx_phase_table_elems(self.period, self.m))
                         if self.use_phase else 0)
         self.K = None
+        self._K32 = None
 
     def key_index(self, key):
         return self.cutoff_index if key < 0 else int(key)
@@ -125,6 +126,17 @@ class PSFTables:
         for i, k in enumerate(missing):
             self.slot_of[k] = base + i
         self.radial_last = radial
+
+    def k32(self):
+        """float32 copy of the phase tables (rebuilt after new keys)."""
+        if not self.use_phase:
+            return None
+        n = len(self.slot_of) * self.k_elems
+        if self._K32 is None or self._K32.numel() != n:
+            self._K32 = torch.empty(n, dtype=torch.float32, device=self.device)
+            _lib.call('smx_phase_table_to_float', self.K.data_ptr(), n, self._K32.data_ptr(),
+                      _device.stream())
+        return self._K32
 
     def slot_map(self):
         """Device int32 table: key index -> slot (-1 if not built)."""

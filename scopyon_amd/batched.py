@@ -28,7 +28,7 @@ class BatchedSimulator:
 
     def __init__(self, config, fov_shapes, n_emitters, *, D=0.1e-12, dt=None, seed=0,
                  boundary='reflect', box=None, depth_range=None, method='default',
-                 adc_dtype=torch.float32, keep_expectation=False):
+                 adc_dtype=torch.float32, keep_expectation=False, table_precision='float32'):
         self.device = _device.require_gpu()
         conf = config[method] if hasattr(config, '__getitem__') else config
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
@@ -122,6 +122,10 @@ class BatchedSimulator:
         self.ws = torch.empty(_lib.lib().smx_splat_workspace_bytes(self.n, self.n_tiles, cap),
                               dtype=torch.uint8, device=self.device)
         self.use_phase = self.tables.use_phase
+        # float32 phase tables halve the splat's table traffic; each pixel
+        # contribution is then rounded to float32 (relative error <= 2^-24)
+        self.k_float = self.use_phase and table_precision == 'float32'
+        self.Ktab = (self.tables.k32() if self.k_float else self.tables.K) if self.use_phase else None
         self.graph = None
 
     # ---- one step ----
@@ -136,7 +140,6 @@ class BatchedSimulator:
     def form(self):
         """Noise-free expectation of every frame (photons per pixel before QE/background)."""
         st = _device.stream()
-        self.photons.zero_()
         _lib.call('smx_emitter_weights', self.n, self.depth.data_ptr(), None,
                   self.splatter.wparams, self.dt, None, None, self.slot_map.data_ptr(),
                   self.slot_map.numel(), self.w.data_ptr(), self.key.data_ptr(), None, st)
@@ -144,8 +147,8 @@ class BatchedSimulator:
         _lib.call('smx_splat_expected', self.n, self.u.data_ptr(), self.v.data_ptr(),
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
                   self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.sat.data_ptr(),
-                  t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 1,
-                  _device.ptr(t.K if self.use_phase else None), t.m, self.photons.data_ptr(),
+                  t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 0,
+                  _device.ptr(self.Ktab), 1 if self.k_float else 0, t.m, self.photons.data_ptr(),
                   self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(), st)
 
     def detect(self):
@@ -154,6 +157,11 @@ class BatchedSimulator:
                   self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
                   self.adc.data_ptr(), 1 if self.adc.dtype == torch.float32 else 0, st)
         _lib.call('smx_counter_add', self.counter.data_ptr(), 1, st)
+
+    def set_positions(self, depth, u, v):
+        """Overwrite the emitter coordinates (concatenated over FOVs)."""
+        for dst, src in ((self.depth, depth), (self.u, u), (self.v, v)):
+            dst.copy_(torch.as_tensor(numpy.ascontiguousarray(src, dtype=numpy.float64)))
 
     def step(self):
         """Advance every FOV by one time step and form + detect its frame."""

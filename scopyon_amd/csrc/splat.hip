@@ -492,6 +492,7 @@ __global__ void phase_table_kernel(const double* __restrict__ sat, size_t sat_st
 }
 
 struct FastArgs {
+    const void* Kraw;        // phase table (double or float)
     const double* u;
     const double* v;
     const int* key;
@@ -540,14 +541,16 @@ __device__ __noinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& F, 
 }
 
 // Each wave owns one 8-pixel-wide column strip of the 32 x 32 tile (four 8 x 8
-// blocks, q = 0..3); an emitter is skipped block by block with wave-uniform
-// tests, so only blocks its footprint touches cost anything.
+// blocks, q = 0..3).  KT is the phase-table element type: double (values
+// identical to the SAT path) or float (half the table traffic; each
+// contribution rounded to float32, relative error <= 2^-24).
+template <typename KT>
 __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     __shared__ int s_sort[SORT_CAP];
     __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (tile-relative), n_u, n_v
     __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
     __shared__ double s_w[FCHUNK];
-    __shared__ const double* s_kb[FCHUNK];     // K block of the emitter's phase
+    __shared__ const KT* s_kb[FCHUNK];         // K block of the emitter's phase
 
     const int tile = blockIdx.x;
     const int fov = A.tile_fov[tile];
@@ -588,40 +591,59 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
             s_geo[threadIdx.x] = make_int4(r.lo_u - tr * TS, r.lo_v - tc * TS, r.n_u, r.n_v);
             s_flags[threadIdx.x] = r.flags;
             s_w[threadIdx.x] = r.w;
-            s_kb[threadIdx.x] = A.K + (r.flags == 0 ? r.table_off : 0);
+            s_kb[threadIdx.x] = (const KT*)A.Kraw + (r.flags == 0 ? r.table_off : 0);
             if (r.flags & 2) atomicOr(A.status, 8);
         }
         __syncthreads();
         if (pj < F.nh) {
-            for (int c = 0; c < nc; ++c) {
-                const int4 g = s_geo[c];
-                const int flags = s_flags[c];
-                if (flags & 2) continue;
-                // this wave's columns [8 wave, 8 wave + 8) against the footprint
-                if (g.y >= wave * 8 + 8 || g.y + g.w <= wave * 8) continue;
-                if (flags == 0) {
-                    const double* kb = s_kb[c];
-                    const double w = s_w[c];
+            // Software-pipelined over EB emitters: every lane issues its EB * PY
+            // table loads unconditionally (address clamped to the block start
+            // on a miss) before any is consumed, with no branch in between.
+            constexpr int EB = 4;
+            for (int c0 = 0; c0 < nc; c0 += EB) {
+                bool irregular = false;
+                for (int b = 0; b < EB && c0 + b < nc; ++b) irregular |= s_flags[c0 + b] == 1;
+                if (irregular) {
+                    // rare (a float64 tie in a cell boundary): ordered scalar path
+                    for (int c = c0; c < min(c0 + EB, nc); ++c) {
+                        const int4 g = s_geo[c];
+#pragma unroll
+                        for (int q = 0; q < PY; ++q) {
+                            int pi = tr * TS + q * 8 + ly;
+                            if (s_flags[c] == 1) {
+                                if (pi < F.nw) acc[q] = sat_pixel(A, F, s_e[c], s_w[c], pi, pj, acc[q]);
+                            } else if (s_flags[c] == 0) {
+                                int du = q * 8 + ly - g.x, dv = cx - g.y;
+                                if ((unsigned)du < (unsigned)g.z && (unsigned)dv < (unsigned)g.w)
+                                    acc[q] += (double)smx::gload(s_kb[c] + du * M + dv) * s_w[c];
+                            }
+                        }
+                    }
+                    continue;
+                }
+                double val[EB][PY];
+#pragma unroll
+                for (int b = 0; b < EB; ++b) {
+                    const int c = min(c0 + b, nc - 1);
+                    const int4 g = s_geo[c];
+                    const bool ok = c0 + b < nc && s_flags[c] == 0;
                     const int dv = cx - g.y;
-                    const bool col_ok = (unsigned)dv < (unsigned)g.w;
-                    double val[PY];
+                    const bool col_ok = ok && (unsigned)dv < (unsigned)g.w;
+                    const KT* kb = s_kb[c];
 #pragma unroll
                     for (int q = 0; q < PY; ++q) {
                         int du = q * 8 + ly - g.x;
                         bool hit = col_ok && (unsigned)du < (unsigned)g.z;
-                        val[q] = smx::gload(kb + (hit ? du * M + dv : 0));
-                        val[q] = hit ? val[q] : 0.0;
+                        double t = (double)smx::gload(kb + (hit ? du * M + dv : 0));
+                        val[b][q] = hit ? t : 0.0;
                     }
+                }
 #pragma unroll
-                    for (int q = 0; q < PY; ++q)
-                        if (g.x < q * 8 + 8 && g.x + g.z > q * 8) acc[q] += val[q] * w;
-                } else {
-                    // irregular boundaries: reference integration through the SAT
+                for (int b = 0; b < EB; ++b) {
+                    const int c = min(c0 + b, nc - 1);
+                    const double w = (c0 + b < nc) ? s_w[c] : 0.0;
 #pragma unroll
-                    for (int q = 0; q < PY; ++q) {
-                        int pi = tr * TS + q * 8 + ly;
-                        if (pi < F.nw) acc[q] = sat_pixel(A, F, s_e[c], s_w[c], pi, pj, acc[q]);
-                    }
+                    for (int q = 0; q < PY; ++q) acc[q] += val[b][q] * w;
                 }
             }
         }
@@ -735,6 +757,8 @@ size_t smx_phase_table_elems(int period, int m) {
     return (size_t)period * period * m * m;
 }
 
+int smx_phase_table_to_float(const double* K, size_t n, float* K32, hipStream_t stream);
+
 int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int ntab,
                           int sat_period, int m, double unit_area, double* K, hipStream_t stream) {
     if (!sat || !K || nslots < 0 || ntab < 1 || sat_period < 2 || m < 1) {
@@ -754,7 +778,7 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
                        const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                        double res, double unit_area, long long list_capacity, int accumulate,
-                       const double* K, int m, double* out, void* workspace,
+                       const void* K, int k_is_float, int m, double* out, void* workspace,
                        size_t workspace_bytes, int* status_dev, hipStream_t stream) {
     if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev || !out ||
         (K && (sat_period < 2 || m < 1)) ||
@@ -810,13 +834,17 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
             SMX_CHECK_LAUNCH("emitter_rec_kernel");
         }
         FastArgs B;
+        B.Kraw = K;
         B.u = u; B.v = v; B.key = key; B.fovs = fovs_dev; B.nfov = nfov;
         B.tile_fov = ws.tile_fov; B.tile_begin = ws.tile_begin; B.list = ws.list;
-        B.capacity = list_capacity; B.rec = ws.rec; B.K = K; B.M = m;
+        B.capacity = list_capacity; B.rec = ws.rec; B.K = (const double*)K; B.M = m;
         B.sat = sat; B.sat_stride = sat_stride; B.L = smx::make_sat_layout(ntab + 1, sat_period);
         B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
         B.accumulate = accumulate; B.out = out; B.status = status_dev;
-        hipLaunchKernelGGL(splat_phase_kernel, dim3(n_tiles), dim3(256), 0, stream, B);
+        if (k_is_float)
+            hipLaunchKernelGGL(splat_phase_kernel<float>, dim3(n_tiles), dim3(256), 0, stream, B);
+        else
+            hipLaunchKernelGGL(splat_phase_kernel<double>, dim3(n_tiles), dim3(256), 0, stream, B);
         SMX_CHECK_LAUNCH("splat_phase_kernel");
         return SMX_OK;
     }
@@ -829,6 +857,23 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
     A.accumulate = accumulate; A.out = out; A.status = status_dev;
     hipLaunchKernelGGL(splat_tile_kernel, dim3(n_tiles), dim3(256), 0, stream, A);
     SMX_CHECK_LAUNCH("splat_tile_kernel");
+    return SMX_OK;
+}
+
+__global__ void to_float_kernel(const double* __restrict__ a, size_t n, float* __restrict__ b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+int smx_phase_table_to_float(const double* K, size_t n, float* K32, hipStream_t stream) {
+    if (!K || !K32) {
+        smx::set_error("smx_phase_table_to_float: null pointer");
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    hipLaunchKernelGGL(to_float_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, K,
+                       n, K32);
+    SMX_CHECK_LAUNCH("to_float_kernel");
     return SMX_OK;
 }
 

@@ -16,11 +16,13 @@ def _cfg(shape=(64, 64)):
                            'effects': {'photo_bleaching': {'switch': False}}})
 
 
-def test_uneven_fovs_match_oracle(gpu):
+@pytest.mark.parametrize('precision,rtol', [('float64', 1e-11), ('float32', 1e-6)])
+def test_uneven_fovs_match_oracle(gpu, precision, rtol):
     from scopyon_amd.batched import BatchedSimulator
     shapes = [(64, 64), (48, 80), (128, 128), (16, 16)]
     counts = [50, 10, 300, 0]
-    sim = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3)
+    sim = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, table_precision=precision)
+    sim.photons.fill_(7.0)      # form() overwrites every pixel
     sim.form()
     assert sim.status() & ~1 == 0
     u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
@@ -34,7 +36,7 @@ def test_uneven_fovs_match_oracle(gpu):
         if k == 0:
             assert not got.any()
         else:
-            numpy.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * ref.max())
+            numpy.testing.assert_allclose(got, ref, rtol=rtol, atol=rtol * 1e-2 * ref.max())
 
 
 def test_graph_replay_equals_eager(gpu):
