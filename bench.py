@@ -68,7 +68,7 @@ def _splat_only_time(sim, reps):
                   sim.w.data_ptr(), sim.key.data_ptr(), sim.fov_id.data_ptr(),
                   sim.fovs_dev.data_ptr(), sim.nfov, sim.n_tiles, t.sat.data_ptr(), t.sat_elems,
                   len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, sim.capacity, 0,
-                  _device.ptr(sim.Ktab), 1 if sim.k_float else 0, t.m, sim.photons.data_ptr(),
+                  _device.ptr(sim.Ktab), sim.k_format, t.m, sim.photons.data_ptr(),
                   sim.ws.data_ptr(), sim.ws.numel(), sim.splatter.status.data_ptr(), st)
     splat()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -88,15 +88,30 @@ int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int 
                           hipStream_t stream);
 /* float32 copy of a phase table (n elements) for the reduced-traffic splat. */
 int smx_phase_table_to_float(const double* K, size_t n, float* K32, hipStream_t stream);
+/* Folded float32 phase tables: the cartesian PSF table is symmetric under
+ * reflection of either axis and under transposition, so pixel integrals of
+ * phase ph equal those of ph' = (ntab - 2 - ph) mod P in reverse pixel order,
+ * and (ph_u, ph_v) equals (ph_v, ph_u) transposed.  One np(0) x np(0) block per
+ * unordered pair of canonical phases keeps every value in ~1/8 of the space
+ * (2.5 MB per depth key at 160 nm pixels: small enough to stay in one XCD's
+ * L2).  smx_phase_fold_host reports the canonical rank of phase ph, whether ph
+ * is the reflected member, and its non-empty pixel count. */
+size_t smx_phase_fold_elems(int ntab, int period);
+int smx_phase_fold_host(int ntab, int period, int ph, int* rank /* host */,
+                        int* reflected /* host */, int* npix /* host */);
+int smx_phase_table_fold(const double* K, int nslots, int ntab, int period, int m, float* Kf,
+                         hipStream_t stream);
 /* Pixel integration of n emitters into nfov frames: replaces
  * PointSpreadingFunction.overlay_signal_ called per emitter
  * (_epifm.py:223-282 via _epifm.py:1261-1264).  u, v: absolute lateral
  * coordinates; w: weights; key: SAT slot per emitter; fov_id: FOV per emitter
  * (NULL: all in FOV 0).  sat: tables of sat_stride doubles each.
  * accumulate=1 adds into `out` (camera += expected); 0 overwrites.
- * K (may be NULL): phase tables from smx_phase_table_build for the same slots
- * (identical results, one load per emitter-pixel instead of four), or their
- * float32 copy when k_is_float (each contribution rounded to float32).
+ * K (may be NULL): phase tables for the same slots, k_format 0: float64 from
+ * smx_phase_table_build (identical results, one load per emitter-pixel
+ * instead of four); 1: its float32 copy (smx_phase_table_to_float); 2: its
+ * folded float32 copy (smx_phase_table_fold).  With 1 and 2 each
+ * contribution is rounded to float32.
  * status bit 1: a tile list exceeded the LDS sort capacity (still correct,
  * order not by emitter index); bit 2: list capacity too small (result
  * incomplete); bit 8: an emitter's table slot was not built (skipped). */
@@ -104,7 +119,7 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
                        const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                        double res, double unit_area, long long list_capacity, int accumulate,
-                       const void* K, int k_is_float, int m, double* out, void* workspace,
+                       const void* K, int k_format, int m, double* out, void* workspace,
                        size_t workspace_bytes, int* status_dev, hipStream_t stream);
 
 /* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */

@@ -68,6 +68,9 @@ SIGNATURES = {
                                    c_int, c_int, c_double, c_double, c_longlong, c_int, P, c_int,
                                    c_int, P, P, c_size_t, P, P]),
     'smx_phase_table_to_float': (c_int, [P, c_size_t, P, P]),
+    'smx_phase_fold_elems': (c_size_t, [c_int, c_int]),
+    'smx_phase_fold_host': (c_int, [c_int, c_int, c_int, P, P, P]),
+    'smx_phase_table_fold': (c_int, [P, c_int, c_int, c_int, c_int, P, P]),
     'smx_detector_expectation': (c_int, [c_int, P, ctypes.POINTER(DetectorParamsT), P, P, P]),
     'smx_scan_workspace_bytes': (c_size_t, [c_int]),
     'smx_exclusive_scan_i32': (c_int, [c_int, P, P, P, c_size_t, P]),

@@ -70,6 +70,9 @@ class PSFTables:
                         if self.use_phase else 0)
         self.K = None
         self._K32 = None
+        self._Kfold = None
+        self.fold_elems = (int(_lib.lib().smx_phase_fold_elems(self.ntab, self.period))
+                           if self.use_phase else 0)
 
     def key_index(self, key):
         return self.cutoff_index if key < 0 else int(key)
@@ -137,6 +140,19 @@ class PSFTables:
             _lib.call('smx_phase_table_to_float', self.K.data_ptr(), n, self._K32.data_ptr(),
                       _device.stream())
         return self._K32
+
+    def kfold(self):
+        """Folded float32 phase tables: one block per unordered pair of
+        canonical phases (about 1/8 of the full table; rebuilt after new keys)."""
+        if not self.use_phase:
+            return None
+        nslots = len(self.slot_of)
+        n = nslots * self.fold_elems
+        if self._Kfold is None or self._Kfold.numel() != n:
+            self._Kfold = torch.empty(n, dtype=torch.float32, device=self.device)
+            _lib.call('smx_phase_table_fold', self.K.data_ptr(), nslots, self.ntab, self.period,
+                      self.m, self._Kfold.data_ptr(), _device.stream())
+        return self._Kfold
 
     def slot_map(self):
         """Device int32 table: key index -> slot (-1 if not built)."""

@@ -28,7 +28,7 @@ class BatchedSimulator:
 
     def __init__(self, config, fov_shapes, n_emitters, *, D=0.1e-12, dt=None, seed=0,
                  boundary='reflect', box=None, depth_range=None, method='default',
-                 adc_dtype=torch.float32, keep_expectation=False, table_precision='float32'):
+                 adc_dtype=torch.float32, keep_expectation=False, table_precision='folded'):
         self.device = _device.require_gpu()
         conf = config[method] if hasattr(config, '__getitem__') else config
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
@@ -122,10 +122,22 @@ class BatchedSimulator:
         self.ws = torch.empty(_lib.lib().smx_splat_workspace_bytes(self.n, self.n_tiles, cap),
                               dtype=torch.uint8, device=self.device)
         self.use_phase = self.tables.use_phase
-        # float32 phase tables halve the splat's table traffic; each pixel
-        # contribution is then rounded to float32 (relative error <= 2^-24)
-        self.k_float = self.use_phase and table_precision == 'float32'
-        self.Ktab = (self.tables.k32() if self.k_float else self.tables.K) if self.use_phase else None
+        # 'float64': the phase tables themselves (same values as the SAT path);
+        # 'float32': their float32 copy (half the table traffic); 'folded': the
+        # symmetric float32 fold (1/8 of the table, L2-resident).  With float32
+        # each pixel contribution is rounded to float32 (relative error <= 2^-24).
+        if table_precision not in ('float64', 'float32', 'folded'):
+            raise ValueError('table_precision must be float64, float32 or folded')
+        self.k_format = 0
+        self.Ktab = None
+        if self.use_phase:
+            self.k_format = {'float64': 0, 'float32': 1, 'folded': 2}[table_precision]
+            if self.k_format == 0:
+                self.Ktab = self.tables.K
+            elif self.k_format == 1:
+                self.Ktab = self.tables.k32()
+            else:
+                self.Ktab = self.tables.kfold()
         self.graph = None
 
     # ---- one step ----
@@ -148,7 +160,7 @@ class BatchedSimulator:
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
                   self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.sat.data_ptr(),
                   t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 0,
-                  _device.ptr(self.Ktab), 1 if self.k_float else 0, t.m, self.photons.data_ptr(),
+                  _device.ptr(self.Ktab), self.k_format, t.m, self.photons.data_ptr(),
                   self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(), st)
 
     def detect(self):

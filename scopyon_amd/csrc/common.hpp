@@ -167,4 +167,51 @@ SMX_HD SatLayout make_sat_layout(int side, int period) {
     return L;
 }
 
+// Symmetry of the phase tables.  Along one axis an emitter of phase ph
+// (0 <= ph < P) has cell boundaries 0, ph+1, ph+1+P, ... clamped to ntab, so
+// its pixel d covers cells [edge(d), edge(d+1)) and there are
+// np(ph) = 1 + ceil((ntab - ph - 1) / P) non-empty pixels.  The cartesian PSF
+// table is symmetric under i -> ntab-1-i, which maps the boundary set of ph
+// onto that of ph' = (ntab - 2 - ph) mod P with the pixels in reverse order:
+// pixel d of ph integrates the same cells as pixel np-1-d of ph'.  The table is
+// also symmetric under transposition, so one block per unordered pair of
+// canonical phases (the smaller of ph, ph') holds every pixel integral.
+struct PhaseFold {
+    int P;      // period (pixel length in cells)
+    int ntab;   // table side in cells
+    int c0;     // (ntab - 2) mod P: ph + ph' == c0 (mod P)
+    int n1;     // canonical phases in [0, c0]: 0 .. c0/2
+    int nc;     // canonical phases in total
+    int mf;     // np(0): the largest pixel count, block side
+    // canonical rank of ph and whether ph is the reflected member of its pair
+    SMX_HD int canon(int ph, bool& reflected) const {
+        int partner = c0 - ph;
+        if (partner < 0) partner += P;
+        reflected = partner < ph;
+        int c = reflected ? partner : ph;
+        return c <= c0 ? c : n1 + (c - c0 - 1);
+    }
+    SMX_HD int uncanon(int rank) const { return rank < n1 ? rank : c0 + 1 + (rank - n1); }
+    SMX_HD int npix(int ph) const {
+        int phi = ph + 1;
+        return phi >= ntab ? 1 : 1 + (ntab - phi + P - 1) / P;
+    }
+    // block of the unordered canonical pair (a <= b)
+    SMX_HD size_t block(int a, int b) const {
+        return (size_t)a * nc - (size_t)a * (a - 1) / 2 + (size_t)(b - a);
+    }
+    SMX_HD size_t elems() const { return (size_t)nc * (nc + 1) / 2 * mf * mf; }
+};
+
+SMX_HD PhaseFold make_phase_fold(int ntab, int P) {
+    PhaseFold f;
+    f.P = P;
+    f.ntab = ntab;
+    f.c0 = ((ntab - 2) % P + P) % P;
+    f.n1 = f.c0 / 2 + 1;
+    f.nc = f.n1 + ((f.c0 + P) / 2 - f.c0);
+    f.mf = f.npix(0);
+    return f;
+}
+
 }  // namespace smx

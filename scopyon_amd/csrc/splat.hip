@@ -10,12 +10,12 @@
 // The cell boundaries B, C are computed with the reference's float64
 // expression ceil((k*pl - o)/res) and the same end clamps.
 //
-// Work decomposition: the frame is cut into 16x16-pixel tiles; emitters are
+// Work decomposition: the frame is cut into 32x32-pixel tiles; emitters are
 // binned to the tiles their footprint touches (count -> scan -> fill), each
 // tile's list is sorted by emitter index in LDS (so every pixel adds its
 // contributions in emitter order, like the reference loop, and the result is
-// run-to-run deterministic), then one workgroup per tile accumulates its 256
-// pixels in registers.  No float atomics.
+// run-to-run deterministic), then one 256-thread workgroup per tile
+// accumulates its 1024 pixels in registers.  No float atomics.
 #include "common.hpp"
 #include "scopyon_hip_types.h"
 
@@ -392,9 +392,10 @@ __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
 struct EmitterRec {
     int lo_u, lo_v;          // first footprint pixel per axis (may be < 0)
     short n_u, n_v;          // footprint pixels per axis (hi - lo)
+    short su, sv;            // table element strides per footprint pixel along u, v
     int flags;               // 1: irregular boundaries -> SAT path; 2: no table slot
     double w;
-    long long table_off;     // offset of K[phi_u][phi_v][0][0]
+    long long table_off;     // element of pixel (du, dv) = table_off + du * su + dv * sv
 };
 
 __device__ inline int regular_phase(const Axis& a, double half, double xi, double ext_half,
@@ -419,12 +420,15 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
                                    const double* __restrict__ w, const int* __restrict__ key,
                                    const int* __restrict__ fov_id, const smx_fov_t* __restrict__ fovs,
                                    int nfov, double ext_half, double res, int ntab, int P, int M,
-                                   int n_slots, size_t k_stride, EmitterRec* __restrict__ rec) {
+                                   int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
+                                   EmitterRec* __restrict__ rec) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     EmitterRec r;
     r.lo_u = r.lo_v = 0;
     r.n_u = r.n_v = 0;
+    r.su = (short)M;
+    r.sv = 1;
     r.flags = 0;
     r.w = w[i];
     r.table_off = 0;
@@ -456,9 +460,32 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
     }
     if (!regular) {
         r.flags = 1;
-    } else {
+    } else if (!folded) {
         r.table_off = (long long)slot * (long long)k_stride +
                       ((long long)ph[0] * P + ph[1]) * (long long)(M * M);
+    } else {
+        // pixels past np(ph) are empty (zero integral): drop them, then address
+        // the canonical block, reversed along a reflected axis, transposed when
+        // the canonical pair is stored as (v, u)
+        const int mf = PF.mf;
+        bool ru, rv;
+        int cu = PF.canon(ph[0], ru), cv = PF.canon(ph[1], rv);
+        int npu = PF.npix(ph[0]), npv = PF.npix(ph[1]);
+        if (r.n_u > npu) r.n_u = (short)npu;
+        if (r.n_v > npv) r.n_v = (short)npv;
+        int a0 = ru ? npu - 1 : 0, sa = ru ? -1 : 1;   // index along u in the block: a0 + sa du
+        int b0 = rv ? npv - 1 : 0, sb = rv ? -1 : 1;
+        long long off;
+        if (cu <= cv) {
+            off = (long long)PF.block(cu, cv) * mf * mf + (long long)a0 * mf + b0;
+            r.su = (short)(sa * mf);
+            r.sv = (short)sb;
+        } else {
+            off = (long long)PF.block(cv, cu) * mf * mf + (long long)b0 * mf + a0;
+            r.su = (short)sa;
+            r.sv = (short)(sb * mf);
+        }
+        r.table_off = (long long)slot * (long long)k_stride + off;
     }
     rec[i] = r;
 }
@@ -491,6 +518,25 @@ __global__ void phase_table_kernel(const double* __restrict__ sat, size_t sat_st
     K[idx] = photons > 0.0 ? photons : 0.0;
 }
 
+// Folded float32 copy: one mf x mf block per unordered pair of canonical
+// phases (smx::PhaseFold), taken from K[slot][ph_u][ph_v][a][b].
+__global__ void phase_fold_kernel(const double* __restrict__ K, int nslots, int P, int M,
+                                  smx::PhaseFold PF, float* __restrict__ Kf) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = PF.nc, mf = PF.mf;
+    size_t per = (size_t)nc * nc * mf * mf;
+    if (idx >= per * nslots) return;
+    size_t slot = idx / per, r = idx % per;
+    int b = (int)(r % mf); r /= mf;
+    int a = (int)(r % mf); r /= mf;
+    int cv = (int)(r % nc), cu = (int)(r / nc);
+    if (cu > cv) return;
+    int pu = PF.uncanon(cu), pv = PF.uncanon(cv);
+    size_t src = (size_t)slot * P * P * M * M + (((size_t)pu * P + pv) * M + a) * M + b;
+    size_t dst = (size_t)slot * PF.elems() + PF.block(cu, cv) * mf * mf + (size_t)a * mf + b;
+    Kf[dst] = (float)K[src];
+}
+
 struct FastArgs {
     const void* Kraw;        // phase table (double or float)
     const double* u;
@@ -515,14 +561,13 @@ struct FastArgs {
     int* status;
 };
 
-// General-path pixel integral for one emitter (used for flagged emitters).
-__device__ __noinline__ double sat_pixel(const struct FastArgs& A, const smx_fov_t& F, int e,
-                                         double w, int pi, int pj, double acc);
 
 constexpr int FCHUNK = 64;
 
-__device__ __noinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& F, int e, double w,
-                                         int pi, int pj, double acc) {
+// General-path pixel integral for one emitter (used for flagged emitters).
+// Inlined: a call would need the kernel argument block in private memory.
+__device__ __forceinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& F, int e, double w,
+                                            int pi, int pj, double acc) {
     int slot = A.key[e];
     Axis au = make_axis(F.half_w, A.u[e] - F.p0u, A.ext_half, F.pixel_length, F.nw);
     Axis av = make_axis(F.half_h, A.v[e] - F.p0v, A.ext_half, F.pixel_length, F.nh);
@@ -540,17 +585,36 @@ __device__ __noinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& F, 
     return acc;
 }
 
-// Each wave owns one 8-pixel-wide column strip of the 32 x 32 tile (four 8 x 8
-// blocks, q = 0..3).  KT is the phase-table element type: double (values
-// identical to the SAT path) or float (half the table traffic; each
-// contribution rounded to float32, relative error <= 2^-24).
+// One wave per 16 x 16 quadrant of the 32 x 32 tile.  Per chunk of the sorted
+// tile list, each wave lists the 64-pixel parts of the emitters' footprints
+// that fall in its quadrant (lane = pixel of the footprint rectangle, so no
+// lane idles on pixels outside it), issues the table loads of NSLOT parts at
+// once, then adds the contributions into its LDS accumulator part by part in
+// list order: every pixel still sums its emitters in emitter order, and the
+// lanes of one part touch distinct pixels.  KT is the phase-table element
+// type: double (values identical to the SAT path) or float (each contribution
+// rounded to float32, relative error <= 2^-24).
+constexpr int QS = 16;            // quadrant side
+constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
+constexpr int PSORT_CAP = 2048;   // tile list sorted in LDS
+
+__device__ inline void lds_add(double* p, double v) {
+    // ds_add_f64: one IEEE add in LDS, in program order per wave
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <typename KT>
 __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
-    __shared__ int s_sort[SORT_CAP];
-    __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (tile-relative), n_u, n_v
+    __shared__ int s_sort[PSORT_CAP];
+    __shared__ double s_acc[4][QS * QS];
+    __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (frame pixels), n_u, n_v
+    __shared__ int2 s_str[FCHUNK];            // table strides per pixel along u, v
     __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
     __shared__ double s_w[FCHUNK];
-    __shared__ const KT* s_kb[FCHUNK];         // K block of the emitter's phase
+    __shared__ const KT* s_kb[FCHUNK];        // table element of footprint pixel (0, 0)
+    __shared__ int4 s_qi[4][FCHUNK];          // per wave: footprint rectangle in the quadrant
+    __shared__ const KT* s_qk[4][FCHUNK];     // per wave: its first table element
+    __shared__ int s_part[4][FCHUNK * 4];     // per wave: emitter << 2 | part
 
     const int tile = blockIdx.x;
     const int fov = A.tile_fov[tile];
@@ -559,15 +623,17 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     const int local = tile - F.tile_off;
     const int tr = local / F.tiles_c, tc = local % F.tiles_c;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ly = lane >> 3, lx = lane & 7;        // position inside an 8 x 8 block
-    const int cx = wave * 8 + lx;                   // tile-relative column
-    const int pj = tc * TS + cx;
+    const int qr0 = tr * TS + (wave >> 1) * QS, qc0 = tc * TS + (wave & 1) * QS;
+    const int qh = min(QS, F.nw - qr0), qw = min(QS, F.nh - qc0);
+    const bool active = qh > 0 && qw > 0;
+    double* acc = s_acc[wave];
+    for (int k = lane; k < QS * QS; k += 64) acc[k] = 0.0;
 
     const int begin = A.tile_begin[tile];
     int count = A.tile_begin[tile + 1] - begin;
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
-    const bool sorted = count <= SORT_CAP;
+    const bool sorted = count <= PSORT_CAP;
     if (count >= 1 && sorted) {
         int np2 = 1;
         while (np2 < count) np2 <<= 1;
@@ -578,87 +644,113 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
         atomicOr(A.status, 1);
     }
 
-    const int M = A.M;
-    double acc[PY];
-#pragma unroll
-    for (int q = 0; q < PY; ++q) acc[q] = 0.0;
     for (int base = 0; base < count; base += FCHUNK) {
         const int nc = min(FCHUNK, count - base);
         if (threadIdx.x < nc) {
             int e = sorted ? s_sort[base + threadIdx.x] : smx::gload(list + base + threadIdx.x);
             EmitterRec r = A.rec[e];
             s_e[threadIdx.x] = e;
-            s_geo[threadIdx.x] = make_int4(r.lo_u - tr * TS, r.lo_v - tc * TS, r.n_u, r.n_v);
+            s_geo[threadIdx.x] = make_int4(r.lo_u, r.lo_v, r.n_u, r.n_v);
+            s_str[threadIdx.x] = make_int2(r.su, r.sv);
             s_flags[threadIdx.x] = r.flags;
             s_w[threadIdx.x] = r.w;
             s_kb[threadIdx.x] = (const KT*)A.Kraw + (r.flags == 0 ? r.table_off : 0);
             if (r.flags & 2) atomicOr(A.status, 8);
         }
         __syncthreads();
-        if (pj < F.nh) {
-            // Software-pipelined over EB emitters: every lane issues its EB * PY
-            // table loads unconditionally (address clamped to the block start
-            // on a miss) before any is consumed, with no branch in between.
-            constexpr int EB = 4;
-            for (int c0 = 0; c0 < nc; c0 += EB) {
-                bool irregular = false;
-                for (int b = 0; b < EB && c0 + b < nc; ++b) irregular |= s_flags[c0 + b] == 1;
-                if (irregular) {
-                    // rare (a float64 tie in a cell boundary): ordered scalar path
-                    for (int c = c0; c < min(c0 + EB, nc); ++c) {
-                        const int4 g = s_geo[c];
+        if (active) {
+            // footprint rectangle of chunk emitter `lane` inside this quadrant:
+            // s_qi = {first pixel r0 * QS + c0, W | magic << 5, area, su | sv << 16},
+            // s_qk = table element of the rectangle's first pixel
+            int np = 0;
+            bool irr = false;
+            if (lane < nc && s_flags[lane] != 2) {
+                const int4 g = s_geo[lane];
+                const int r0 = max(g.x, qr0), r1 = min(g.x + g.z, qr0 + qh);
+                const int c0 = max(g.y, qc0), c1 = min(g.y + g.w, qc0 + qw);
+                if (r1 > r0 && c1 > c0) {
+                    const int W = c1 - c0, area = (r1 - r0) * W;
+                    const int2 st = s_str[lane];
+                    np = (area + 63) >> 6;
+                    irr = s_flags[lane] == 1;
+                    s_qi[wave][lane] = make_int4((r0 - qr0) * QS + (c0 - qc0),
+                                                 W | (((65536 + W - 1) / W) << 5), area,
+                                                 (int)(((unsigned)st.x & 0xffffu) | ((unsigned)st.y << 16)));
+                    s_qk[wave][lane] = s_kb[lane] + (r0 - g.x) * st.x + (c0 - g.y) * st.y;
+                }
+            }
+            const bool any_irregular = __ballot(irr) != 0;
+            int incl = np;
 #pragma unroll
-                        for (int q = 0; q < PY; ++q) {
-                            int pi = tr * TS + q * 8 + ly;
-                            if (s_flags[c] == 1) {
-                                if (pi < F.nw) acc[q] = sat_pixel(A, F, s_e[c], s_w[c], pi, pj, acc[q]);
-                            } else if (s_flags[c] == 0) {
-                                int du = q * 8 + ly - g.x, dv = cx - g.y;
-                                if ((unsigned)du < (unsigned)g.z && (unsigned)dv < (unsigned)g.w)
-                                    acc[q] += (double)smx::gload(s_kb[c] + du * M + dv) * s_w[c];
-                            }
+            for (int d = 1; d < 64; d <<= 1) {
+                int t = __shfl_up(incl, d);
+                if (lane >= d) incl += t;
+            }
+            const int total = __shfl(incl, 63);
+            for (int p = 0; p < np; ++p) s_part[wave][incl - np + p] = (lane << 2) | p;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+            if (any_irregular) {
+                // rare (a float64 tie in a cell boundary): one part at a time, in
+                // order, flagged emitters integrated through the SAT
+                for (int b = 0; b < total; ++b) {
+                    const int ent = s_part[wave][b], c = ent >> 2;
+                    const int4 I = s_qi[wave][c];
+                    const int idx = (ent & 3) * 64 + lane;
+                    if (idx >= I.z) continue;
+                    const int W = I.y & 31;
+                    const int row = idx / W, col = idx - row * W;
+                    const int pq = I.x + row * QS + col;
+                    double val;
+                    if (s_flags[c] == 1) {
+                        val = sat_pixel(A, F, s_e[c], s_w[c], qr0 + (pq / QS), qc0 + (pq % QS), 0.0);
+                    } else {
+                        const int su = (short)(I.w & 0xffff), sv = I.w >> 16;
+                        val = (double)smx::gload(s_qk[wave][c] + row * su + col * sv) * s_w[c];
+                    }
+                    lds_add(acc + pq, val);
+                }
+            } else {
+                for (int b = 0; b < total; b += NSLOT) {
+                    KT kv[NSLOT];
+                    double wv[NSLOT];
+                    int pix[NSLOT];
+#pragma unroll
+                    for (int s = 0; s < NSLOT; ++s) {
+                        pix[s] = -1;
+                        if (b + s < total) {
+                            const int ent = s_part[wave][b + s], c = ent >> 2;
+                            const int4 I = s_qi[wave][c];
+                            const int idx = (ent & 3) * 64 + lane;
+                            const bool ok = idx < I.z;
+                            const int W = I.y & 31, magic = I.y >> 5;
+                            const int row = (idx * magic) >> 16, col = idx - row * W;
+                            const int su = (short)(I.w & 0xffff), sv = I.w >> 16;
+                            kv[s] = smx::gload(s_qk[wave][c] + (ok ? row * su + col * sv : 0));
+                            wv[s] = s_w[c];
+                            pix[s] = ok ? I.x + row * QS + col : -1;
                         }
                     }
-                    continue;
-                }
-                double val[EB][PY];
 #pragma unroll
-                for (int b = 0; b < EB; ++b) {
-                    const int c = min(c0 + b, nc - 1);
-                    const int4 g = s_geo[c];
-                    const bool ok = c0 + b < nc && s_flags[c] == 0;
-                    const int dv = cx - g.y;
-                    const bool col_ok = ok && (unsigned)dv < (unsigned)g.w;
-                    const KT* kb = s_kb[c];
-#pragma unroll
-                    for (int q = 0; q < PY; ++q) {
-                        int du = q * 8 + ly - g.x;
-                        bool hit = col_ok && (unsigned)du < (unsigned)g.z;
-                        double t = (double)smx::gload(kb + (hit ? du * M + dv : 0));
-                        val[b][q] = hit ? t : 0.0;
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < EB; ++b) {
-                    const int c = min(c0 + b, nc - 1);
-                    const double w = (c0 + b < nc) ? s_w[c] : 0.0;
-#pragma unroll
-                    for (int q = 0; q < PY; ++q) acc[q] += val[b][q] * w;
+                    for (int s = 0; s < NSLOT; ++s)
+                        if (pix[s] >= 0) lds_add(acc + pix[s], (double)kv[s] * wv[s]);
                 }
             }
         }
         __syncthreads();
     }
-    if (pj < F.nh) {
-#pragma unroll
-        for (int q = 0; q < PY; ++q) {
-            int pi = tr * TS + q * 8 + ly;
-            if (pi >= F.nw) continue;
-            size_t o = (size_t)F.pix_off + (size_t)pi * F.nh + pj;
+    if (active) {
+        for (int k = lane; k < QS * QS; k += 64) {
+            const int r = k / QS, c = k % QS;
+            if (r >= qh || c >= qw) continue;
+            const size_t o = (size_t)F.pix_off + (size_t)(qr0 + r) * F.nh + (qc0 + c);
             if (A.accumulate) {
-                if (count > 0) A.out[o] = A.out[o] + acc[q];
+                if (count > 0) A.out[o] = A.out[o] + acc[k];
             } else {
-                A.out[o] = acc[q];
+                // streaming store: keep the phase table, not the frame, in L2
+                __builtin_nontemporal_store(acc[k], A.out + o);
             }
         }
     }
@@ -778,10 +870,12 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
                        const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                        double res, double unit_area, long long list_capacity, int accumulate,
-                       const void* K, int k_is_float, int m, double* out, void* workspace,
+                       const void* K, int k_format, int m, double* out, void* workspace,
                        size_t workspace_bytes, int* status_dev, hipStream_t stream) {
+    const smx::PhaseFold PF = smx::make_phase_fold(ntab, sat_period > 0 ? sat_period : 1);
     if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev || !out ||
-        (K && (sat_period < 2 || m < 1)) ||
+        (K && (sat_period < 2 || m < 1 || k_format < 0 || k_format > 2)) ||
+        (K && k_format == 2 && PF.mf > m) ||
         !workspace || !status_dev || (n > 0 && (!u || !v || !w || !key || !sat))) {
         smx::set_error("smx_splat_expected: invalid arguments (n=%d, nfov=%d, n_tiles=%d)", n,
                        nfov, n_tiles);
@@ -828,9 +922,11 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
     }
     if (K) {
         if (n > 0) {
+            const int folded = k_format == 2;
             hipLaunchKernelGGL(emitter_rec_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, key,
                                fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m, n_slots,
-                               smx_phase_table_elems(sat_period, m), ws.rec);
+                               folded ? PF.elems() : smx_phase_table_elems(sat_period, m), folded,
+                               PF, ws.rec);
             SMX_CHECK_LAUNCH("emitter_rec_kernel");
         }
         FastArgs B;
@@ -841,7 +937,7 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
         B.sat = sat; B.sat_stride = sat_stride; B.L = smx::make_sat_layout(ntab + 1, sat_period);
         B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
         B.accumulate = accumulate; B.out = out; B.status = status_dev;
-        if (k_is_float)
+        if (k_format != 0)
             hipLaunchKernelGGL(splat_phase_kernel<float>, dim3(n_tiles), dim3(256), 0, stream, B);
         else
             hipLaunchKernelGGL(splat_phase_kernel<double>, dim3(n_tiles), dim3(256), 0, stream, B);
@@ -874,6 +970,46 @@ int smx_phase_table_to_float(const double* K, size_t n, float* K32, hipStream_t 
     hipLaunchKernelGGL(to_float_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, K,
                        n, K32);
     SMX_CHECK_LAUNCH("to_float_kernel");
+    return SMX_OK;
+}
+
+
+size_t smx_phase_fold_elems(int ntab, int period) {
+    if (ntab < 1 || period < 2) return 0;
+    return smx::make_phase_fold(ntab, period).elems();
+}
+
+int smx_phase_fold_host(int ntab, int period, int ph, int* rank, int* reflected, int* npix) {
+    if (ntab < 1 || period < 2 || ph < 0 || ph >= period) {
+        smx::set_error("smx_phase_fold_host: invalid arguments (ntab=%d, period=%d, ph=%d)", ntab,
+                       period, ph);
+        return SMX_EVALUE;
+    }
+    smx::PhaseFold f = smx::make_phase_fold(ntab, period);
+    bool r = false;
+    int c = f.canon(ph, r);
+    if (rank) *rank = c;
+    if (reflected) *reflected = r ? 1 : 0;
+    if (npix) *npix = f.npix(ph);
+    return SMX_OK;
+}
+
+int smx_phase_table_fold(const double* K, int nslots, int ntab, int period, int m, float* Kf,
+                         hipStream_t stream) {
+    if (!K || !Kf || nslots < 0 || ntab < 1 || period < 2 || m < 1) {
+        smx::set_error("smx_phase_table_fold: invalid arguments (period=%d, m=%d)", period, m);
+        return SMX_EVALUE;
+    }
+    smx::PhaseFold f = smx::make_phase_fold(ntab, period);
+    if (f.mf > m) {
+        smx::set_error("smx_phase_table_fold: block side %d exceeds the table's %d", f.mf, m);
+        return SMX_EVALUE;
+    }
+    if (nslots == 0) return SMX_OK;
+    size_t total = (size_t)f.nc * f.nc * f.mf * f.mf * (size_t)nslots;
+    hipLaunchKernelGGL(phase_fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, K, nslots, period, m, f, Kf);
+    SMX_CHECK_LAUNCH("phase_fold_kernel");
     return SMX_OK;
 }
 

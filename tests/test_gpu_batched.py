@@ -16,7 +16,8 @@ def _cfg(shape=(64, 64)):
                            'effects': {'photo_bleaching': {'switch': False}}})
 
 
-@pytest.mark.parametrize('precision,rtol', [('float64', 1e-11), ('float32', 1e-6)])
+@pytest.mark.parametrize('precision,rtol', [('float64', 1e-11), ('float32', 1e-6),
+                                            ('folded', 1e-6)])
 def test_uneven_fovs_match_oracle(gpu, precision, rtol):
     from scopyon_amd.batched import BatchedSimulator
     shapes = [(64, 64), (48, 80), (128, 128), (16, 16)]

@@ -84,3 +84,42 @@ def test_bad_arguments_raise():
         _lib.call('smx_psf_radial_gaussian', 1, 1e-9, -1.0, None, None)
     msg = lib.smx_last_error().decode()
     assert 'smx_psf_radial_gaussian' in msg
+
+
+def _pixel_cells(ntab, period, ph):
+    """Cell ranges [a, b) of the pixels of phase ph: boundaries 0, ph+1,
+    ph+1+P, ... clamped to ntab (the regular grid of _epifm.py:241-263)."""
+    edges = [0] + [min(ph + 1 + period * k, ntab) for k in range(ntab // period + 3)]
+    cells = [(a, b) for a, b in zip(edges[:-1], edges[1:]) if b > a]
+    return cells
+
+
+@pytest.mark.parametrize('ntab,period', [(1999, 160), (1999, 65), (1999, 2), (101, 10),
+                                         (101, 7), (5, 8)])
+def test_phase_fold_matches_reflection_of_cell_ranges(ntab, period):
+    """The folded phase tables rest on one fact: reflecting the table
+    (cell i -> ntab-1-i) maps the pixel cell ranges of phase ph onto those of
+    its partner in reverse order.  Checked by brute force on the ranges, with
+    the canonical ranks forming a bijection onto the orbits {ph, partner}."""
+    lib = _lib.lib()
+    rank, refl, npix = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    orbits = {}
+    mf = 0
+    for ph in range(period):
+        _lib.call('smx_phase_fold_host', ntab, period, ph, ctypes.byref(rank), ctypes.byref(refl),
+                  ctypes.byref(npix))
+        cells = _pixel_cells(ntab, period, ph)
+        assert npix.value == len(cells)
+        mf = max(mf, npix.value)
+        reflected = sorted((ntab - b, ntab - a) for a, b in cells)
+        partners = [q for q in range(period) if _pixel_cells(ntab, period, q) == reflected]
+        partner = (ntab - 2 - ph) % period
+        # several phases share one range set only when the pixel exceeds the table
+        assert partner in partners and (len(partners) == 1 or period >= ntab)
+        assert refl.value == (partner < ph)
+        orbits.setdefault(rank.value, set()).update({ph, partner})
+    assert sorted(orbits) == list(range(len(orbits)))
+    assert all(len(o) in (1, 2) for o in orbits.values())
+    assert sum(len(o) for o in orbits.values()) == period
+    nc = len(orbits)
+    assert lib.smx_phase_fold_elems(ntab, period) == nc * (nc + 1) // 2 * mf * mf
