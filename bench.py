@@ -3,17 +3,21 @@
 Workload (the metric's configuration): F fields of view, each a 512 x 512
 EMCCD frame (16 um pixels, 100x magnification -> 160 nm) with 10,000 emitters
 diffusing in 2D (D = 0.1 um^2/s, dt = 33 ms, reflecting walls at the frame
-edge), TRITC Born-Wolf PSF integrated per pixel from the float64 summed-area
-table, EM gain 300, readout sigma 100 e-, 16-bit ADC.  One step = Brownian step
-+ emitter weights + splat + EMCCD/ADC for every frame of the batch, replayed
-from one captured HIP graph.  Inputs are synthetic and resident in HBM.
+edge), TRITC Born-Wolf PSF integrated per pixel (phase tables built from the
+float64 summed-area table), EM gain 300, readout sigma 100 e-, 16-bit ADC.  One
+step = Brownian step + emitter weights + splat + EMCCD/ADC for every frame of
+the batch, replayed from one captured HIP graph.  Inputs are synthetic and
+resident in HBM.
 
 Also reported: the plain single-frame path (same kernels, one frame per step,
 eager launches), the drop-in form_image() path (RandomState stream,
-reference-exact EMCCD), per-stage kernel times from HIP events, the roofline of
-the dominant kernel, and the CPU baseline (the float64 oracle on this host).
+reference-exact EMCCD), per-kernel times from HIP events, the roofline of the
+dominant kernel (traffic from the committed rocprofv3 --pmc summary), the
+differentiable forward+backward rate, and the CPU baseline (the float64 oracle
+on this host, one core).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--fovs F]
+    python bench.py --calibrate-copy      (copy kernel for counter scaling)
 """
 import argparse
 import json
@@ -38,13 +42,20 @@ def _config(size, exposure):
     return cfg
 
 
-def _stage_times(sim, reps):
-    """Average duration (ms) of each stage of an eager step, from HIP events
-    on the launch stream."""
+def _kernel_times(sim, reps):
+    """Average duration (ms) of each kernel stage of one batched step, from HIP
+    events recorded on the stream the kernels are launched on."""
     import torch
-    stages = {'brownian_philox': sim.move, 'weights+splat': sim.form, 'emccd_philox': sim.detect}
+    stages = {
+        'brownian_philox': sim.move,
+        'emitter_weights': sim.weights,
+        'splat_bin': lambda: sim.splat('smx_splat_bin'),
+        'splat_accumulate': lambda: sim.splat('smx_splat_accumulate'),
+        'emccd_philox': sim.emccd,
+    }
     out = {}
     for name, fn in stages.items():
+        fn()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(reps)]
         for a, b in evs:
@@ -56,28 +67,44 @@ def _stage_times(sim, reps):
     return out
 
 
-def _splat_only_time(sim, reps):
-    """Average duration (ms) of the splat call alone (binning + tile kernel)."""
-    import torch
-    from scopyon_amd import _lib, _device
-    t = sim.tables
-    st = _device.stream()
+def _stage_bytes(N, HW):
+    """Algorithmic HBM bytes per launch of each stage (N emitters, HW pixels)."""
+    return {
+        # 2D float64 coordinates read + written (32 B) and the FOV id (4 B)
+        'brownian_philox': N * 36,
+        # depth read (8 B); weight (8 B) and table slot (4 B) written
+        'emitter_weights': N * 20,
+        # u, v, w read (24 B); one 40 B table record written
+        'splat_bin': N * 64,
+        # emitter position and weight (24 B); the float64 photon image written (8 B/px)
+        'splat_accumulate': N * 24 + HW * 8,
+        # float64 photons read (8 B/px), float32 ADC written (4 B/px)
+        'emccd_philox': HW * 12,
+    }
 
-    def splat():
-        _lib.call('smx_splat_expected', sim.n, sim.u.data_ptr(), sim.v.data_ptr(),
-                  sim.w.data_ptr(), sim.key.data_ptr(), sim.fov_id.data_ptr(),
-                  sim.fovs_dev.data_ptr(), sim.nfov, sim.n_tiles, t.sat.data_ptr(), t.sat_elems,
-                  len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, sim.capacity, 0,
-                  _device.ptr(sim.Ktab), sim.k_format, t.m, sim.photons.data_ptr(),
-                  sim.ws.data_ptr(), sim.ws.numel(), sim.splatter.status.data_ptr(), st)
-    splat()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
+
+def _traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data['kernels'][kernel]['hbm_bytes_per_launch']
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def calibrate_copy(n_bytes=1 << 28, reps=5):
+    """A plain device copy of known size, for scaling the FETCH_SIZE/WRITE_SIZE
+    counters: each launch reads and writes n_bytes."""
+    import torch
+    a = torch.ones(n_bytes // 8, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
     for _ in range(reps):
-        splat()
-    b.record()
+        b.copy_(a)
     torch.cuda.synchronize()
-    return a.elapsed_time(b) / reps
+    print(json.dumps({'copy_bytes_read': n_bytes, 'copy_bytes_written': n_bytes, 'reps': reps}))
 
 
 def cpu_baseline(size, n_emit, sample_emitters, seed=1):
@@ -142,12 +169,17 @@ def main():
     ap.add_argument('--plain-steps', type=int, default=20)
     ap.add_argument('--cpu-sample', type=int, default=200)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='only run a device copy of known size (for rocprofv3 --pmc scaling)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only; --gpus {} is not supported (the FOV batch '
-              'would shard across GPUs with no communication, see DESIGN.md)'.format(args.gpus),
+              'would shard across GPUs with no communication)'.format(args.gpus),
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import numpy
     import torch
     import scopyon_amd
@@ -193,21 +225,11 @@ def main():
         scopyon_amd.form_image(inputs, config=cfg, rng=rng)
     api_fps = napi / (time.perf_counter() - t0)
 
-    # per-stage times (HIP events on the launch stream) and the dominant kernel
-    stages = _stage_times(sim, 10)
-    splat_ms = _splat_only_time(sim, 10)
-    N, HW = sim.n, sim.n_pix
-    stage_bytes = {
-        # coords read+write (2 x 16 B) + fov_id 4 B + box per emitter
-        'brownian_philox': N * 36,
-        # depth read 8 B, weight 8 B + key 4 B written; splat re-reads u, v, w, key
-        # (28 B) and read-modify-writes every float64 pixel (16 B)
-        'weights+splat': N * 20 + N * 28 + HW * 16,
-        # photons read 8 B, float32 ADC written 4 B
-        'emccd_philox': HW * 12,
-    }
-    dom = max(stages, key=stages.get)
-    achieved = stage_bytes[dom] / (stages[dom] * 1e-3) / 1e9
+    # per-kernel times (HIP events on the launch stream) and the dominant kernel
+    kt = _kernel_times(sim, 10)
+    sbytes = _stage_bytes(sim.n, sim.n_pix)
+    dom = max(kt, key=kt.get)
+    achieved = sbytes[dom] / (kt[dom] * 1e-3) / 1e9
     result = {
         'metric': METRIC,
         'value': batched_fps,
@@ -224,16 +246,21 @@ def main():
         'config': {'workload': 'batched TIRF EMCCD frames, {0}x{0} px, {1} emitters per FOV, '
                                '{2} FOVs per step, hipGraph replay'.format(S, args.emitters, F),
                    'fovs': F, 'frame': [S, S], 'emitters_per_fov': args.emitters,
-                   'pixel_nm': 160, 'psf': 'TRITC Born-Wolf, float64 SAT', 'dt_s': dt,
-                   'D_um2_s': 0.1, 'walls': 'reflecting', 'emccd': 'philox'},
+                   'pixel_nm': 160,
+                   'psf': 'TRITC Born-Wolf; folded float32 phase table from the float64 SAT; '
+                          'float64 accumulation',
+                   'dt_s': dt, 'D_um2_s': 0.1, 'walls': 'reflecting',
+                   'emccd': 'Philox4x32-10, float32 sampler'},
+        # plain = the same kernels, one frame per step, eager launches
         'plain_frames_per_sec': plain_fps,
         'batched_over_plain': batched_fps / plain_fps,
+        # drop-in form_image(): RandomState draws on the host, reference-exact EMCCD
         'api_form_image_frames_per_sec': api_fps,
-        'stage_ms': stages,
-        'splat_call_ms': splat_ms,
+        'batched_over_api_form_image': batched_fps / api_fps,
+        'kernel_ms': kt,
         'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'algorithmic_bytes': stage_bytes[dom]},
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': _traffic(dom),
+                     'algorithmic_bytes': sbytes[dom], 'launch_ms': kt[dom]},
     }
     if not args.no_cpu:
         base, rel = cpu_baseline(S, args.emitters, args.cpu_sample)

@@ -121,6 +121,23 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                        double res, double unit_area, long long list_capacity, int accumulate,
                        const void* K, int k_format, int m, double* out, void* workspace,
                        size_t workspace_bytes, int* status_dev, hipStream_t stream);
+/* The two halves of smx_splat_expected, same arguments: smx_splat_bin bins
+ * the emitters to tiles (and builds their table records) in `workspace`;
+ * smx_splat_accumulate integrates the tiles from that workspace into `out`.
+ * Lets a caller time or repeat the tile pass on its own. */
+int smx_splat_bin(int n, const double* u, const double* v, const double* w, const int* key,
+                  const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
+                  const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
+                  double res, double unit_area, long long list_capacity, int accumulate,
+                  const void* K, int k_format, int m, double* out, void* workspace,
+                  size_t workspace_bytes, int* status_dev, hipStream_t stream);
+int smx_splat_accumulate(int n, const double* u, const double* v, const double* w,
+                         const int* key, const int* fov_id, const smx_fov_t* fovs_dev, int nfov,
+                         int n_tiles, const double* sat, size_t sat_stride, int n_slots, int ntab,
+                         int sat_period, double res, double unit_area, long long list_capacity,
+                         int accumulate, const void* K, int k_format, int m, double* out,
+                         void* workspace, size_t workspace_bytes, int* status_dev,
+                         hipStream_t stream);
 
 /* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */
 /* E = QE * (photons + background) (_epifm.py:1436-1441); flag_out (may be

@@ -68,6 +68,7 @@ SIGNATURES = {
                                    c_int, c_int, c_double, c_double, c_longlong, c_int, P, c_int,
                                    c_int, P, P, c_size_t, P, P]),
     'smx_phase_table_to_float': (c_int, [P, c_size_t, P, P]),
+
     'smx_phase_fold_elems': (c_size_t, [c_int, c_int]),
     'smx_phase_fold_host': (c_int, [c_int, c_int, c_int, P, P, P]),
     'smx_phase_table_fold': (c_int, [P, c_int, c_int, c_int, c_int, P, P]),
@@ -88,6 +89,10 @@ SIGNATURES = {
 }
 
 _lib = None
+
+
+SIGNATURES['smx_splat_bin'] = SIGNATURES['smx_splat_expected']
+SIGNATURES['smx_splat_accumulate'] = SIGNATURES['smx_splat_expected']
 
 
 def lib():

@@ -151,24 +151,38 @@ class BatchedSimulator:
 
     def form(self):
         """Noise-free expectation of every frame (photons per pixel before QE/background)."""
-        st = _device.stream()
+        self.weights()
+        self.splat()
+
+    def weights(self):
+        """Per-emitter weight and depth-table slot (_epifm.py:1280-1315)."""
         _lib.call('smx_emitter_weights', self.n, self.depth.data_ptr(), None,
                   self.splatter.wparams, self.dt, None, None, self.slot_map.data_ptr(),
-                  self.slot_map.numel(), self.w.data_ptr(), self.key.data_ptr(), None, st)
+                  self.slot_map.numel(), self.w.data_ptr(), self.key.data_ptr(), None,
+                  _device.stream())
+
+    def splat(self, entry='smx_splat_expected'):
+        """Pixel integration of every FOV (entry: smx_splat_expected, or its two
+        halves smx_splat_bin / smx_splat_accumulate)."""
         t = self.tables
-        _lib.call('smx_splat_expected', self.n, self.u.data_ptr(), self.v.data_ptr(),
+        _lib.call(entry, self.n, self.u.data_ptr(), self.v.data_ptr(),
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
                   self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.sat.data_ptr(),
                   t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 0,
                   _device.ptr(self.Ktab), self.k_format, t.m, self.photons.data_ptr(),
-                  self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(), st)
+                  self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(),
+                  _device.stream())
 
     def detect(self):
-        st = _device.stream()
+        """EMCCD + ADC of every frame, then advance the frame counter."""
+        self.emccd()
+        _lib.call('smx_counter_add', self.counter.data_ptr(), 1, _device.stream())
+
+    def emccd(self):
         _lib.call('smx_emccd_philox', self.n_pix, self.photons.data_ptr(), None, None, self.det,
                   self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
-                  self.adc.data_ptr(), 1 if self.adc.dtype == torch.float32 else 0, st)
-        _lib.call('smx_counter_add', self.counter.data_ptr(), 1, st)
+                  self.adc.data_ptr(), 1 if self.adc.dtype == torch.float32 else 0,
+                  _device.stream())
 
     def set_positions(self, depth, u, v):
         """Overwrite the emitter coordinates (concatenated over FOVs)."""

@@ -866,12 +866,14 @@ int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int 
     return SMX_OK;
 }
 
-int smx_splat_expected(int n, const double* u, const double* v, const double* w, const int* key,
-                       const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
-                       const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
-                       double res, double unit_area, long long list_capacity, int accumulate,
-                       const void* K, int k_format, int m, double* out, void* workspace,
-                       size_t workspace_bytes, int* status_dev, hipStream_t stream) {
+// stages: 1 = bin the emitters to tiles (and, with K, their table records),
+// 2 = accumulate the tiles from the binned workspace, 3 = both.
+static int splat_run(int n, const double* u, const double* v, const double* w, const int* key,
+                     const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
+                     const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
+                     double res, double unit_area, long long list_capacity, int accumulate,
+                     const void* K, int k_format, int m, double* out, void* workspace,
+                     size_t workspace_bytes, int* status_dev, hipStream_t stream, int stages) {
     const smx::PhaseFold PF = smx::make_phase_fold(ntab, sat_period > 0 ? sat_period : 1);
     if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev || !out ||
         (K && (sat_period < 2 || m < 1 || k_format < 0 || k_format > 2)) ||
@@ -892,36 +894,36 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
     const double ext_half = (res * (double)(ntab - 1)) * 0.5;   // signal_width * 0.5
     const int nb = (n + 255) / 256;
 
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0, stream,
-                       ws.tile_count, (size_t)n_tiles + 1, 0);
-    SMX_CHECK_LAUNCH("fill_int_kernel");
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
-                       ws.tile_cursor, (size_t)n_tiles, 0);
-    SMX_CHECK_LAUNCH("fill_int_kernel");
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
-                       ws.tile_fov, (size_t)n_tiles, -1);
-    SMX_CHECK_LAUNCH("fill_int_kernel");
-    hipLaunchKernelGGL(tile_fov_kernel, dim3(nfov), dim3(256), 0, stream, nfov, fovs_dev,
-                       ws.tile_fov, n_tiles);
-    SMX_CHECK_LAUNCH("tile_fov_kernel");
-    if (n > 0) {
-        hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, fov_id,
-                           fovs_dev, nfov, ext_half, n_tiles, ws.tile_count);
-        SMX_CHECK_LAUNCH("bin_count_kernel");
-    }
-    size_t cb = ws.cub_bytes;
-    if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_count,
-                                                        ws.tile_begin, n_tiles + 1, stream),
-                       "hipcub ExclusiveSum"))
-        return SMX_ERUNTIME;
-    if (n > 0) {
-        hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, fov_id,
-                           fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin, ws.tile_cursor, ws.list,
-                           list_capacity, status_dev);
-        SMX_CHECK_LAUNCH("bin_fill_kernel");
-    }
-    if (K) {
+    if (stages & 1) {
+        hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0, stream,
+                           ws.tile_count, (size_t)n_tiles + 1, 0);
+        SMX_CHECK_LAUNCH("fill_int_kernel");
+        hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
+                           ws.tile_cursor, (size_t)n_tiles, 0);
+        SMX_CHECK_LAUNCH("fill_int_kernel");
+        hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
+                           ws.tile_fov, (size_t)n_tiles, -1);
+        SMX_CHECK_LAUNCH("fill_int_kernel");
+        hipLaunchKernelGGL(tile_fov_kernel, dim3(nfov), dim3(256), 0, stream, nfov, fovs_dev,
+                           ws.tile_fov, n_tiles);
+        SMX_CHECK_LAUNCH("tile_fov_kernel");
         if (n > 0) {
+            hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, fov_id,
+                               fovs_dev, nfov, ext_half, n_tiles, ws.tile_count);
+            SMX_CHECK_LAUNCH("bin_count_kernel");
+        }
+        size_t cb = ws.cub_bytes;
+        if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_count,
+                                                            ws.tile_begin, n_tiles + 1, stream),
+                           "hipcub ExclusiveSum"))
+            return SMX_ERUNTIME;
+        if (n > 0) {
+            hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, fov_id,
+                               fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin, ws.tile_cursor, ws.list,
+                               list_capacity, status_dev);
+            SMX_CHECK_LAUNCH("bin_fill_kernel");
+        }
+        if (K && n > 0) {
             const int folded = k_format == 2;
             hipLaunchKernelGGL(emitter_rec_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, key,
                                fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m, n_slots,
@@ -929,6 +931,9 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
                                PF, ws.rec);
             SMX_CHECK_LAUNCH("emitter_rec_kernel");
         }
+    }
+    if (!(stages & 2)) return SMX_OK;
+    if (K) {
         FastArgs B;
         B.Kraw = K;
         B.u = u; B.v = v; B.key = key; B.fovs = fovs_dev; B.nfov = nfov;
@@ -955,6 +960,21 @@ int smx_splat_expected(int n, const double* u, const double* v, const double* w,
     SMX_CHECK_LAUNCH("splat_tile_kernel");
     return SMX_OK;
 }
+
+#define SMX_SPLAT_PARAMS                                                                          \
+    int n, const double *u, const double *v, const double *w, const int *key, const int *fov_id, \
+        const smx_fov_t *fovs_dev, int nfov, int n_tiles, const double *sat, size_t sat_stride,   \
+        int n_slots, int ntab, int sat_period, double res, double unit_area,                      \
+        long long list_capacity, int accumulate, const void *K, int k_format, int m, double *out,  \
+        void *workspace, size_t workspace_bytes, int *status_dev, hipStream_t stream
+#define SMX_SPLAT_ARGS                                                                       \
+    n, u, v, w, key, fov_id, fovs_dev, nfov, n_tiles, sat, sat_stride, n_slots, ntab,         \
+        sat_period, res, unit_area, list_capacity, accumulate, K, k_format, m, out, workspace, \
+        workspace_bytes, status_dev, stream
+
+int smx_splat_expected(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 3); }
+int smx_splat_bin(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 1); }
+int smx_splat_accumulate(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 2); }
 
 __global__ void to_float_kernel(const double* __restrict__ a, size_t n, float* __restrict__ b) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
