@@ -169,6 +169,41 @@ int smx_emccd_philox(int n, const double* photons, const double* gain_map,
                      const uint32_t* frame_dev, double* E_out, void* adc_out,
                      int adc_is_float32, hipStream_t stream);
 
+/* ---- separable Gaussian model: forward, Poisson NLL, backward ----------- */
+/* Frames reuse the smx_fov_t fields nw, nh, tiles_r, tiles_c, pix_off and
+ * tile_off (the length fields are unused: coordinates are pixels).  Emitter e belongs to
+ * frame frame_id[e] (NULL: frame 0).  The model is documented at
+ * smx_gauss_params_t; the Gaussian profile is _epifm.py:128-134 and the depth
+ * factor the TIRF amplitude of _epifm.py:1289.  Float32 throughout; the
+ * per-frame loss and the PSF-width gradient are reduced in float64. */
+size_t smx_gauss_workspace_bytes(int n, int n_tiles, long long capacity);
+/* mu = background + sum of emitter contributions (every pixel of every frame
+ * is written).  capacity: tile-list entries (each emitter enters at most four
+ * 32 x 32 tiles).  status bit 1: a tile list was too long to sort in LDS
+ * (result correct, summation order not by emitter index); bit 2: capacity too
+ * small (result incomplete). */
+int smx_gauss_forward(int n, const float* x, const float* y, const float* z, const float* I,
+                      const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                      const smx_gauss_params_t* params /* host */, long long capacity, float* mu,
+                      void* workspace, size_t workspace_bytes, int* status_dev,
+                      hipStream_t stream);
+/* loss[f] = sum over frame f of mu - k log(mu) (float64); grad_mu (may be NULL)
+ * = 1 - k / mu.  workspace: smx_gauss_nll_workspace_bytes(nframes, max pixels
+ * of any frame). */
+size_t smx_gauss_nll_workspace_bytes(int nframes, int max_frame_pixels);
+int smx_gauss_poisson_nll(const float* mu, const float* k, const smx_fov_t* frames_dev,
+                          int nframes, int max_frame_pixels, double* loss, float* grad_mu,
+                          void* workspace, size_t workspace_bytes, hipStream_t stream);
+/* Hand-written backward of smx_gauss_forward: given grad_mu = dL/dmu, writes
+ * dL/dx, dL/dy, dL/dz, dL/dI per emitter (any output may be NULL) and, when
+ * gsigma (device scalar) is not NULL, dL/dsigma summed over all emitters in a
+ * fixed order (float64); workspace then holds at least 8 * n bytes. */
+int smx_gauss_backward(int n, const float* x, const float* y, const float* z, const float* I,
+                       const int* frame_id, const smx_fov_t* frames_dev, int nframes,
+                       const smx_gauss_params_t* params /* host */, const float* grad_mu,
+                       float* gx, float* gy, float* gz, float* gI, double* gsigma,
+                       void* workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* ---- Brownian sampling (sampling2.py:33-68, sampling.py:88-119) --------- */
 /* Whole trajectories from caller-drawn normals xi[t][i][d] (and transition
  * uniforms trans_u[t][i]): x += scale[t][state][d] * xi; boundary 0 none,

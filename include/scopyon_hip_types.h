@@ -15,7 +15,7 @@ extern "C" {
  * image_size[1] (v) (src/scopyon/_epifm.py:1175-1177). */
 typedef struct smx_fov_t {
     int32_t nw, nh;          /* image_size = (nw, nh) */
-    int32_t tiles_r, tiles_c;/* ceil(nw / 16), ceil(nh / 16) */
+    int32_t tiles_r, tiles_c;/* ceil(nw / 32), ceil(nh / 32): smx_splat_tile_size() */
     int64_t pix_off;         /* first pixel of this frame in the batch buffer */
     int32_t tile_off;        /* first global tile id of this frame */
     int32_t reserved;
@@ -53,6 +53,22 @@ typedef struct smx_detector_params_t {
     double offset;             /* scalar ADC offset (used when offset_map is NULL) */
     double gain;               /* scalar ADC gain (used when gain_map is NULL) */
 } smx_detector_params_t;
+
+/* Separable Gaussian image model in pixel units (the differentiable and the
+ * matrix-core forms of the splat).  Emitter e at (x, y) (pixel i covers
+ * [i, i + 1)), depth z and intensity I adds to pixel (i, j) of its frame
+ *   I * exp(-|z| * inv_depth) * G(i; x) * G(j; y),
+ *   G(i; x) = (erf((i + 1 - x) / (sqrt(2) sigma)) - erf((i - x) / (sqrt(2) sigma))) / 2,
+ * over the window i in [floor(x) - radius, floor(x) + radius] (the Gaussian
+ * radial profile of _epifm.py:128-134 integrated over each pixel; the depth
+ * factor is the TIRF amplitude exp(-|depth| / penetration_depth) of
+ * _epifm.py:1289).  The expectation is background + the sum over emitters. */
+typedef struct smx_gauss_params_t {
+    float sigma;        /* PSF width [px] */
+    float inv_depth;    /* 1 / penetration depth [1/um]; 0 disables the depth factor */
+    float background;   /* expected photons per pixel added to every frame */
+    int32_t radius;     /* window half-width R (1 <= R <= 7): 2R + 1 pixels per axis */
+} smx_gauss_params_t;
 
 #ifdef __cplusplus
 }

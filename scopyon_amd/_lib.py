@@ -35,6 +35,12 @@ class WeightParamsT(ctypes.Structure):
                 ('reserved', ctypes.c_int32)]
 
 
+class GaussParamsT(ctypes.Structure):
+    """smx_gauss_params_t."""
+    _fields_ = [('sigma', ctypes.c_float), ('inv_depth', ctypes.c_float),
+                ('background', ctypes.c_float), ('radius', ctypes.c_int32)]
+
+
 class DetectorParamsT(ctypes.Structure):
     """smx_detector_params_t."""
     _fields_ = [('qe', c_double), ('background', c_double), ('background_switch', ctypes.c_int32),
@@ -91,6 +97,15 @@ SIGNATURES = {
 _lib = None
 
 
+SIGNATURES.update({
+    'smx_gauss_workspace_bytes': (c_size_t, [c_int, c_int, c_longlong]),
+    'smx_gauss_forward': (c_int, [c_int, P, P, P, P, P, P, c_int, c_int,
+                                  ctypes.POINTER(GaussParamsT), c_longlong, P, P, c_size_t, P, P]),
+    'smx_gauss_nll_workspace_bytes': (c_size_t, [c_int, c_int]),
+    'smx_gauss_poisson_nll': (c_int, [P, P, P, c_int, c_int, P, P, P, c_size_t, P]),
+    'smx_gauss_backward': (c_int, [c_int, P, P, P, P, P, P, c_int, ctypes.POINTER(GaussParamsT),
+                                   P, P, P, P, P, P, P, c_size_t, P]),
+})
 SIGNATURES['smx_splat_bin'] = SIGNATURES['smx_splat_expected']
 SIGNATURES['smx_splat_accumulate'] = SIGNATURES['smx_splat_expected']
 

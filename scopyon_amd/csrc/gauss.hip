@@ -1,0 +1,564 @@
+// Separable Gaussian image model: forward image, Poisson negative
+// log-likelihood with its image gradient, and the hand-written backward with
+// respect to emitter positions, depths, intensities and the PSF width.
+//
+// Reference: the Gaussian radial profile exp(-r^2 / (2 w^2)) / (2 pi w^2) of
+// src/scopyon/_epifm.py:128-134 (depth independent), integrated over each
+// pixel; the TIRF amplitude exp(-|depth| / penetration_depth) of
+// _epifm.py:1289.  Units here are pixels (x, y, sigma) and micrometres (z).
+//
+// Forward: emitters are binned to 32 x 32 tiles, each tile's list is sorted by
+// emitter index, one wave per 16 x 16 quadrant adds every emitter's
+// contributions in emitter order (deterministic, no float atomics in global
+// memory).  The per-axis pixel factors G(i; x) are evaluated once per emitter
+// and tile chunk into LDS (2 x (2R + 1) erf differences); a contribution is
+// then one product.
+//
+// Backward: one 16-lane group per emitter.  With g = dL/dmu over the
+// emitter's (2R+1)^2 window, every gradient is a separable sum
+//   sum_ij g_ij u_i v_j  (u, v in {G, dG/dx, dG/dsigma}),
+// so each lane owns one window column, walks the rows (one coalesced load
+// per row), and the group reduces five dot products with shuffles.
+#include "common.hpp"
+#include "scopyon_hip_types.h"
+#include "tiles.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace {
+
+using smx::align256;
+using smx::fill_int_kernel;
+using smx::lds_add;
+using smx::lds_bitonic_sort;
+using smx::tile_fov_kernel;
+
+constexpr int TS = 32;           // tile side
+constexpr int QS = 16;           // quadrant side (one wave)
+constexpr int FCHUNK = 64;       // emitters staged per round
+constexpr int PSORT_CAP = 2048;  // tile list sorted in LDS
+constexpr int WMAX = 15;         // largest window (radius 7)
+constexpr int NSLOT = 8;         // 64-pixel parts in flight per wave
+constexpr float INV_SQRT2 = 0.70710678118654752f;
+constexpr float INV_SQRT2PI = 0.39894228040143268f;
+
+// Pixel-integrated unit Gaussian along one axis: (erf(t1) - erf(t0)) / 2 with
+// t = (edge - x) / (sqrt(2) sigma); the complementary form keeps the tails
+// accurate (no difference of two numbers close to +-1).
+__device__ inline float g_pixel(float i, float x, float s) {
+    float t0 = (i - x) * s, t1 = (i + 1.0f - x) * s;
+    if (t0 >= 0.0f) return 0.5f * (erfcf(t0) - erfcf(t1));
+    if (t1 <= 0.0f) return 0.5f * (erfcf(-t1) - erfcf(-t0));
+    return 0.5f * (erff(t1) - erff(t0));
+}
+
+// Normal density N(t) = exp(-t^2 / (2 sigma^2)) / (sqrt(2 pi) sigma).
+__device__ inline float normal_pdf(float t, float inv_sigma) {
+    float u = t * inv_sigma;
+    return INV_SQRT2PI * inv_sigma * expf(-0.5f * u * u);
+}
+
+__device__ inline bool finite_coord(float v) { return v > -1.0e6f && v < 1.0e9f; }
+
+struct TileRange { int r0, r1, c0, c1; };
+
+// Window rows [floor(x) - R, floor(x) + R] and columns likewise, clipped to
+// the frame; false when the window misses the frame.
+__device__ inline bool window_tiles(const smx_fov_t& F, float x, float y, int R, TileRange& t) {
+    if (!finite_coord(x) || !finite_coord(y)) return false;
+    int lu = (int)floorf(x) - R, lv = (int)floorf(y) - R;
+    int r0 = max(lu, 0), r1 = min(lu + 2 * R, F.nw - 1);
+    int c0 = max(lv, 0), c1 = min(lv + 2 * R, F.nh - 1);
+    if (r1 < r0 || c1 < c0) return false;
+    t.r0 = r0 / TS; t.r1 = r1 / TS; t.c0 = c0 / TS; t.c1 = c1 / TS;
+    return true;
+}
+
+__device__ inline int frame_of(const int* __restrict__ frame_id, int i) { return frame_id ? frame_id[i] : 0; }
+
+__global__ void gauss_bin_count_kernel(int n, const float* __restrict__ x, const float* __restrict__ y,
+                                       const int* __restrict__ frame_id,
+                                       const smx_fov_t* __restrict__ frames, int nframes, int R,
+                                       int n_tiles, int* __restrict__ tile_count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int f = frame_of(frame_id, i);
+    if (f < 0 || f >= nframes) return;
+    const smx_fov_t F = frames[f];
+    TileRange t;
+    if (!window_tiles(F, x[i], y[i], R, t)) return;
+    for (int r = t.r0; r <= t.r1; ++r)
+        for (int c = t.c0; c <= t.c1; ++c) {
+            int tile = F.tile_off + r * F.tiles_c + c;
+            if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[tile], 1);
+        }
+}
+
+__global__ void gauss_bin_fill_kernel(int n, const float* __restrict__ x, const float* __restrict__ y,
+                                      const int* __restrict__ frame_id,
+                                      const smx_fov_t* __restrict__ frames, int nframes, int R,
+                                      int n_tiles, const int* __restrict__ tile_begin,
+                                      int* __restrict__ tile_cursor, int* __restrict__ list,
+                                      long long capacity, int* __restrict__ status) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int f = frame_of(frame_id, i);
+    if (f < 0 || f >= nframes) return;
+    const smx_fov_t F = frames[f];
+    TileRange t;
+    if (!window_tiles(F, x[i], y[i], R, t)) return;
+    for (int r = t.r0; r <= t.r1; ++r)
+        for (int c = t.c0; c <= t.c1; ++c) {
+            int tile = F.tile_off + r * F.tiles_c + c;
+            if (tile < 0 || tile >= n_tiles) continue;
+            int pos = atomicAdd(&tile_cursor[tile], 1);
+            long long at = (long long)tile_begin[tile] + pos;
+            if (at < capacity) list[at] = i;
+            else atomicOr(status, 2);
+        }
+}
+
+struct FwdArgs {
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* I;
+    const smx_fov_t* frames;
+    int nframes;
+    const int* tile_fov;
+    const int* tile_begin;
+    const int* list;
+    long long capacity;
+    smx_gauss_params_t P;
+    float* mu;
+    int* status;
+};
+
+__global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
+    __shared__ int s_sort[PSORT_CAP];
+    __shared__ float s_acc[4][QS * QS];
+    __shared__ float s_g[FCHUNK][2][16];       // pixel factors along u (rows) and v
+    __shared__ int2 s_lo[FCHUNK];              // window origin (frame pixels)
+    __shared__ float2 s_xy[FCHUNK];
+    __shared__ float s_amp[FCHUNK];
+    __shared__ int4 s_qi[4][FCHUNK];           // per wave: rectangle in the quadrant
+    __shared__ int s_part[4][FCHUNK * 4];      // per wave: emitter << 2 | part
+
+    const int tile = blockIdx.x;
+    const int fr = A.tile_fov[tile];
+    if (fr < 0 || fr >= A.nframes) return;
+    const smx_fov_t F = A.frames[fr];
+    const int local = tile - F.tile_off;
+    const int tr = local / F.tiles_c, tc = local % F.tiles_c;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int qr0 = tr * TS + (wave >> 1) * QS, qc0 = tc * TS + (wave & 1) * QS;
+    const int qh = min(QS, F.nw - qr0), qw = min(QS, F.nh - qc0);
+    const bool active = qh > 0 && qw > 0;
+    const int R = A.P.radius, W = 2 * R + 1;
+    const float s = INV_SQRT2 / A.P.sigma;
+    float* acc = s_acc[wave];
+    for (int k = lane; k < QS * QS; k += 64) acc[k] = 0.0f;
+
+    const int begin = A.tile_begin[tile];
+    int count = A.tile_begin[tile + 1] - begin;
+    if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
+    const int* list = A.list + begin;
+    const bool sorted = count <= PSORT_CAP;
+    if (count >= 1 && sorted) {
+        int np2 = 1;
+        while (np2 < count) np2 <<= 1;
+        for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
+        __syncthreads();
+        if (count > 1) lds_bitonic_sort(s_sort, np2);
+    } else if (!sorted && threadIdx.x == 0) {
+        atomicOr(A.status, 1);
+    }
+
+    for (int base = 0; base < count; base += FCHUNK) {
+        const int nc = min(FCHUNK, count - base);
+        if (threadIdx.x < nc) {
+            const int e = sorted ? s_sort[base + threadIdx.x] : list[base + threadIdx.x];
+            const float xe = A.x[e], ye = A.y[e];
+            s_xy[threadIdx.x] = make_float2(xe, ye);
+            s_lo[threadIdx.x] = make_int2((int)floorf(xe) - R, (int)floorf(ye) - R);
+            s_amp[threadIdx.x] = A.I[e] * expf(-fabsf(A.z[e]) * A.P.inv_depth);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < nc * 2 * W; t += blockDim.x) {
+            const int c = t / (2 * W), r = t - c * (2 * W);
+            const int ax = r >= W, k = ax ? r - W : r;
+            const float pos = ax ? s_xy[c].y : s_xy[c].x;
+            const int lo = ax ? s_lo[c].y : s_lo[c].x;
+            s_g[c][ax][k] = g_pixel((float)(lo + k), pos, s);
+        }
+        __syncthreads();
+        if (active) {
+            // rectangle of chunk emitter `lane` inside this quadrant:
+            // {first pixel, W | magic << 5, area, (row offset | col offset << 8) in the window}
+            int np = 0;
+            if (lane < nc) {
+                const int2 lo = s_lo[lane];
+                const int r0 = max(lo.x, qr0), r1 = min(lo.x + W, qr0 + qh);
+                const int c0 = max(lo.y, qc0), c1 = min(lo.y + W, qc0 + qw);
+                if (r1 > r0 && c1 > c0) {
+                    const int Wr = c1 - c0, area = (r1 - r0) * Wr;
+                    np = (area + 63) >> 6;
+                    s_qi[wave][lane] = make_int4((r0 - qr0) * QS + (c0 - qc0),
+                                                 Wr | (((65536 + Wr - 1) / Wr) << 5), area,
+                                                 (r0 - lo.x) | ((c0 - lo.y) << 8));
+                }
+            }
+            int incl = np;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                int t = __shfl_up(incl, d);
+                if (lane >= d) incl += t;
+            }
+            const int total = __shfl(incl, 63);
+            for (int p = 0; p < np; ++p) s_part[wave][incl - np + p] = (lane << 2) | p;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int b = 0; b < total; b += NSLOT) {
+                float val[NSLOT];
+                int pix[NSLOT];
+#pragma unroll
+                for (int q = 0; q < NSLOT; ++q) {
+                    pix[q] = -1;
+                    val[q] = 0.0f;
+                    if (b + q < total) {
+                        const int ent = s_part[wave][b + q], c = ent >> 2;
+                        const int4 Q = s_qi[wave][c];
+                        const int idx = (ent & 3) * 64 + lane;
+                        if (idx < Q.z) {
+                            const int Wr = Q.y & 31, magic = Q.y >> 5;
+                            const int row = (idx * magic) >> 16, col = idx - row * Wr;
+                            const int ru = (Q.w & 255) + row, cv = (Q.w >> 8) + col;
+                            val[q] = (s_amp[c] * s_g[c][0][ru]) * s_g[c][1][cv];
+                            pix[q] = Q.x + row * QS + col;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NSLOT; ++q)
+                    if (pix[q] >= 0) lds_add(acc + pix[q], val[q]);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+        for (int k = lane; k < QS * QS; k += 64) {
+            const int r = k / QS, c = k % QS;
+            if (r >= qh || c >= qw) continue;
+            const size_t o = (size_t)F.pix_off + (size_t)(qr0 + r) * F.nh + (qc0 + c);
+            A.mu[o] = A.P.background + acc[k];
+        }
+    }
+}
+
+// Poisson NLL per frame, sum over pixels of mu - k log(mu) (float64), and
+// dL/dmu = 1 - k / mu.  Block b of frame f reduces pixels [b * per, (b+1) * per).
+__global__ void __launch_bounds__(256) gauss_nll_kernel(const float* __restrict__ mu,
+                                                        const float* __restrict__ k,
+                                                        const smx_fov_t* __restrict__ frames,
+                                                        int per, double* __restrict__ partial,
+                                                        float* __restrict__ grad) {
+    __shared__ double s_red[256];
+    const int f = blockIdx.y, nblk = gridDim.x;
+    const smx_fov_t F = frames[f];
+    const long long npix = (long long)F.nw * F.nh;
+    const long long start = (long long)blockIdx.x * per;
+    const long long stop = min(start + (long long)per, npix);
+    double sum = 0.0;
+    for (long long p = start + threadIdx.x; p < stop; p += blockDim.x) {
+        const size_t o = (size_t)F.pix_off + (size_t)p;
+        const float m = mu[o], kk = k[o];
+        const double md = (double)(m > 0.0f ? m : 1.0e-30f);
+        sum += md - (double)kk * log(md);
+        if (grad) grad[o] = 1.0f - kk / (m > 0.0f ? m : 1.0e-30f);
+    }
+    s_red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)f * nblk + blockIdx.x] = s_red[0];
+}
+
+__global__ void gauss_nll_finish_kernel(const double* __restrict__ partial, int nblk, int nframes,
+                                        double* __restrict__ loss) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(size_t)f * nblk + b];
+    loss[f] = s;
+}
+
+struct BwdArgs {
+    int n;
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* I;
+    const int* frame_id;
+    const smx_fov_t* frames;
+    int nframes;
+    smx_gauss_params_t P;
+    const float* grad_mu;
+    float* gx;
+    float* gy;
+    float* gz;
+    float* gI;
+    double* gsig_part;   // per emitter: dL/dsigma contribution (may be NULL)
+};
+
+__global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
+    const int e = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int j = threadIdx.x & 15;
+    const int gl = (threadIdx.x & 63) & ~15;    // first lane of this emitter's group
+    const int R = A.P.radius, W = 2 * R + 1;
+    const float sigma = A.P.sigma, inv_sigma = 1.0f / sigma, s = INV_SQRT2 * inv_sigma;
+    bool valid = e < A.n;
+    const int f = valid ? frame_of(A.frame_id, e) : 0;
+    valid = valid && f >= 0 && f < A.nframes;
+    smx_fov_t F;
+    F.nw = F.nh = 0;
+    F.pix_off = 0;
+    float xe = 0.0f, ye = 0.0f, ze = 0.0f, Ie = 0.0f;
+    if (valid) {
+        F = A.frames[f];
+        xe = A.x[e];
+        ye = A.y[e];
+        ze = A.z[e];
+        Ie = A.I[e];
+        valid = finite_coord(xe) && finite_coord(ye);
+    }
+    const int lu = valid ? (int)floorf(xe) - R : 0, lv = valid ? (int)floorf(ye) - R : 0;
+    // this lane's window row factors (row lu + j) and column factors (col lv + j)
+    float gxr = 0.0f, dgxr = 0.0f, sgxr = 0.0f, gyc = 0.0f, dgyc = 0.0f, sgyc = 0.0f;
+    if (valid && j < W) {
+        const float i0 = (float)(lu + j), j0 = (float)(lv + j);
+        gxr = g_pixel(i0, xe, s);
+        gyc = g_pixel(j0, ye, s);
+        const float a0 = i0 - xe, a1 = a0 + 1.0f, b0 = j0 - ye, b1 = b0 + 1.0f;
+        const float na0 = normal_pdf(a0, inv_sigma), na1 = normal_pdf(a1, inv_sigma);
+        const float nb0 = normal_pdf(b0, inv_sigma), nb1 = normal_pdf(b1, inv_sigma);
+        dgxr = na0 - na1;                               // dG/dx
+        dgyc = nb0 - nb1;
+        sgxr = -(a1 * na1 - a0 * na0) * inv_sigma;      // dG/dsigma
+        sgyc = -(b1 * nb1 - b0 * nb0) * inv_sigma;
+    }
+    const int col = lv + j;
+    const bool col_in = valid && j < W && col >= 0 && col < F.nh;
+    float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+    for (int k = 0; k < W; ++k) {
+        const float gk = __shfl(gxr, gl + k), dk = __shfl(dgxr, gl + k), sk = __shfl(sgxr, gl + k);
+        const int row = lu + k;
+        float g = 0.0f;
+        if (col_in && row >= 0 && row < F.nw)
+            g = A.grad_mu[(size_t)F.pix_off + (size_t)row * F.nh + col];
+        t1 += g * gk;
+        t2 += g * dk;
+        t3 += g * sk;
+    }
+    float pI = t1 * gyc, px = t2 * gyc, py = t1 * dgyc, ps = t3 * gyc + t1 * sgyc;
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) {
+        pI += __shfl_xor(pI, m);
+        px += __shfl_xor(px, m);
+        py += __shfl_xor(py, m);
+        ps += __shfl_xor(ps, m);
+    }
+    if (j == 0 && e < A.n) {
+        const float a = valid ? expf(-fabsf(ze) * A.P.inv_depth) : 0.0f;
+        const float Ia = Ie * a;
+        const float sgn = ze > 0.0f ? 1.0f : (ze < 0.0f ? -1.0f : 0.0f);
+        if (A.gI) A.gI[e] = a * pI;
+        if (A.gx) A.gx[e] = Ia * px;
+        if (A.gy) A.gy[e] = Ia * py;
+        if (A.gz) A.gz[e] = -sgn * A.P.inv_depth * Ia * pI;
+        if (A.gsig_part) A.gsig_part[e] = (double)(Ia * ps);
+    }
+}
+
+// Deterministic float64 sum of n values by one workgroup (fixed assignment).
+__global__ void __launch_bounds__(1024) sum_f64_kernel(const double* __restrict__ in, int n,
+                                                       double* __restrict__ out) {
+    __shared__ double s_red[1024];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += in[i];
+    s_red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s_red[0];
+}
+
+struct GaussWorkspace {
+    int* tile_count;   // n_tiles + 1
+    int* tile_begin;   // n_tiles + 1
+    int* tile_cursor;  // n_tiles
+    int* tile_fov;     // n_tiles
+    int* list;         // capacity
+    double* gsig;      // n
+    void* cub_tmp;
+    size_t cub_bytes;
+};
+
+size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWorkspace* ws) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    size_t o_count = take(sizeof(int) * (n_tiles + 1));
+    size_t o_begin = take(sizeof(int) * (n_tiles + 1));
+    size_t o_cursor = take(sizeof(int) * n_tiles);
+    size_t o_fov = take(sizeof(int) * n_tiles);
+    size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
+    size_t o_sig = take(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    size_t cb = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cb, (int*)nullptr, (int*)nullptr, n_tiles + 1);
+    size_t o_cub = take(cb);
+    if (ws && base) {
+        ws->tile_count = (int*)(base + o_count);
+        ws->tile_begin = (int*)(base + o_begin);
+        ws->tile_cursor = (int*)(base + o_cursor);
+        ws->tile_fov = (int*)(base + o_fov);
+        ws->list = (int*)(base + o_list);
+        ws->gsig = (double*)(base + o_sig);
+        ws->cub_tmp = base + o_cub;
+        ws->cub_bytes = cb;
+    }
+    return off;
+}
+
+bool params_ok(const smx_gauss_params_t* P) {
+    return P && P->sigma > 0.0f && P->radius >= 1 && 2 * P->radius + 1 <= WMAX &&
+           P->inv_depth >= 0.0f;
+}
+
+constexpr int NLL_PER = 4096;
+
+}  // namespace
+
+extern "C" {
+
+size_t smx_gauss_workspace_bytes(int n, int n_tiles, long long capacity) {
+    if (n < 0 || n_tiles <= 0 || capacity < 0) return 0;
+    return gauss_layout(n, n_tiles, capacity, nullptr, nullptr);
+}
+
+int smx_gauss_forward(int n, const float* x, const float* y, const float* z, const float* I,
+                      const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                      const smx_gauss_params_t* params, long long capacity, float* mu,
+                      void* workspace, size_t workspace_bytes, int* status_dev,
+                      hipStream_t stream) {
+    if (n < 0 || nframes <= 0 || n_tiles <= 0 || !frames_dev || !mu || !workspace ||
+        !status_dev || !params_ok(params) || (n > 0 && (!x || !y || !z || !I))) {
+        smx::set_error("smx_gauss_forward: invalid arguments (n=%d, nframes=%d, n_tiles=%d)", n,
+                       nframes, n_tiles);
+        return SMX_EVALUE;
+    }
+    size_t need = gauss_layout(n, n_tiles, capacity, nullptr, nullptr);
+    if (workspace_bytes < need) {
+        smx::set_error("smx_gauss_forward: workspace too small (%zu < %zu bytes)", workspace_bytes,
+                       need);
+        return SMX_EVALUE;
+    }
+    GaussWorkspace ws;
+    gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
+    const int R = params->radius;
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0, stream,
+                       ws.tile_count, (size_t)n_tiles + 1, 0);
+    SMX_CHECK_LAUNCH("fill_int_kernel");
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
+                       ws.tile_cursor, (size_t)n_tiles, 0);
+    SMX_CHECK_LAUNCH("fill_int_kernel");
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
+                       ws.tile_fov, (size_t)n_tiles, -1);
+    SMX_CHECK_LAUNCH("fill_int_kernel");
+    hipLaunchKernelGGL(tile_fov_kernel, dim3(nframes), dim3(256), 0, stream, nframes, frames_dev,
+                       ws.tile_fov, n_tiles);
+    SMX_CHECK_LAUNCH("tile_fov_kernel");
+    if (n > 0) {
+        hipLaunchKernelGGL(gauss_bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, x, y,
+                           frame_id, frames_dev, nframes, R, n_tiles, ws.tile_count);
+        SMX_CHECK_LAUNCH("gauss_bin_count_kernel");
+    }
+    size_t cb = ws.cub_bytes;
+    if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_count,
+                                                        ws.tile_begin, n_tiles + 1, stream),
+                       "hipcub ExclusiveSum"))
+        return SMX_ERUNTIME;
+    if (n > 0) {
+        hipLaunchKernelGGL(gauss_bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, x, y,
+                           frame_id, frames_dev, nframes, R, n_tiles, ws.tile_begin,
+                           ws.tile_cursor, ws.list, capacity, status_dev);
+        SMX_CHECK_LAUNCH("gauss_bin_fill_kernel");
+    }
+    FwdArgs A;
+    A.x = x; A.y = y; A.z = z; A.I = I; A.frames = frames_dev; A.nframes = nframes;
+    A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
+    A.capacity = capacity; A.P = *params; A.mu = mu; A.status = status_dev;
+    hipLaunchKernelGGL(gauss_forward_kernel, dim3(n_tiles), dim3(256), 0, stream, A);
+    SMX_CHECK_LAUNCH("gauss_forward_kernel");
+    return SMX_OK;
+}
+
+size_t smx_gauss_nll_workspace_bytes(int nframes, int max_frame_pixels) {
+    if (nframes <= 0 || max_frame_pixels <= 0) return 0;
+    int nblk = (max_frame_pixels + NLL_PER - 1) / NLL_PER;
+    return sizeof(double) * (size_t)nblk * nframes;
+}
+
+int smx_gauss_poisson_nll(const float* mu, const float* k, const smx_fov_t* frames_dev,
+                          int nframes, int max_frame_pixels, double* loss, float* grad_mu,
+                          void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    if (!mu || !k || !frames_dev || nframes <= 0 || max_frame_pixels <= 0 || !loss ||
+        !workspace || workspace_bytes < smx_gauss_nll_workspace_bytes(nframes, max_frame_pixels)) {
+        smx::set_error("smx_gauss_poisson_nll: invalid arguments (nframes=%d)", nframes);
+        return SMX_EVALUE;
+    }
+    int nblk = (max_frame_pixels + NLL_PER - 1) / NLL_PER;
+    hipLaunchKernelGGL(gauss_nll_kernel, dim3(nblk, nframes), dim3(256), 0, stream, mu, k,
+                       frames_dev, NLL_PER, (double*)workspace, grad_mu);
+    SMX_CHECK_LAUNCH("gauss_nll_kernel");
+    hipLaunchKernelGGL(gauss_nll_finish_kernel, dim3((nframes + 255) / 256), dim3(256), 0, stream,
+                       (const double*)workspace, nblk, nframes, loss);
+    SMX_CHECK_LAUNCH("gauss_nll_finish_kernel");
+    return SMX_OK;
+}
+
+int smx_gauss_backward(int n, const float* x, const float* y, const float* z, const float* I,
+                       const int* frame_id, const smx_fov_t* frames_dev, int nframes,
+                       const smx_gauss_params_t* params, const float* grad_mu, float* gx,
+                       float* gy, float* gz, float* gI, double* gsigma, void* workspace,
+                       size_t workspace_bytes, hipStream_t stream) {
+    if (n < 0 || nframes <= 0 || !frames_dev || !grad_mu || !params_ok(params) ||
+        (n > 0 && (!x || !y || !z || !I)) ||
+        (gsigma && (!workspace || workspace_bytes < sizeof(double) * (size_t)(n > 0 ? n : 1)))) {
+        smx::set_error("smx_gauss_backward: invalid arguments (n=%d, nframes=%d)", n, nframes);
+        return SMX_EVALUE;
+    }
+    if (n == 0) {
+        if (gsigma) hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, stream,
+                                       (const double*)workspace, 0, gsigma);
+        return SMX_OK;
+    }
+    BwdArgs B;
+    B.n = n; B.x = x; B.y = y; B.z = z; B.I = I; B.frame_id = frame_id; B.frames = frames_dev;
+    B.nframes = nframes; B.P = *params; B.grad_mu = grad_mu; B.gx = gx; B.gy = gy; B.gz = gz;
+    B.gI = gI; B.gsig_part = gsigma ? (double*)workspace : nullptr;
+    hipLaunchKernelGGL(gauss_backward_kernel, dim3((n + 15) / 16), dim3(256), 0, stream, B);
+    SMX_CHECK_LAUNCH("gauss_backward_kernel");
+    if (gsigma) {
+        hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, stream,
+                           (const double*)workspace, n, gsigma);
+        SMX_CHECK_LAUNCH("sum_f64_kernel");
+    }
+    return SMX_OK;
+}
+
+}  // extern "C"

@@ -18,10 +18,17 @@
 // accumulates its 1024 pixels in registers.  No float atomics.
 #include "common.hpp"
 #include "scopyon_hip_types.h"
+#include "tiles.hpp"
 
 #include <hipcub/hipcub.hpp>
 
 namespace {
+
+using smx::align256;
+using smx::fill_int_kernel;
+using smx::lds_add;
+using smx::lds_bitonic_sort;
+using smx::tile_fov_kernel;
 
 constexpr int TS = 32;         // tile side in pixels
 constexpr int TB = TS + 1;     // boundaries per tile axis
@@ -159,11 +166,6 @@ __global__ void bin_fill_kernel(int n, const double* __restrict__ u, const doubl
         }
 }
 
-__global__ void fill_int_kernel(int* __restrict__ p, size_t n, int value) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = value;
-}
-
 // --------------------------------------------------------------------------
 // Tile accumulation
 // --------------------------------------------------------------------------
@@ -189,25 +191,6 @@ struct TileArgs {
     double* out;
     int* status;             // set to 1 if a tile list had to be processed unsorted
 };
-
-__device__ void lds_bitonic_sort(int* a, int n_pow2) {
-    for (int k = 2; k <= n_pow2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n_pow2; i += blockDim.x) {
-                int ixj = i ^ j;
-                if (ixj > i) {
-                    int x = a[i], y = a[ixj];
-                    bool up = (i & k) == 0;
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ixj] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
 
 __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
     // LDS: the sort buffer is only live before the main loop; the corner
@@ -598,11 +581,6 @@ constexpr int QS = 16;            // quadrant side
 constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
 constexpr int PSORT_CAP = 2048;   // tile list sorted in LDS
 
-__device__ inline void lds_add(double* p, double v) {
-    // ds_add_f64: one IEEE add in LDS, in program order per wave
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
 template <typename KT>
 __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     __shared__ int s_sort[PSORT_CAP];
@@ -756,18 +734,6 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     }
 }
 
-__global__ void tile_fov_kernel(int nfov, const smx_fov_t* __restrict__ fovs, int* __restrict__ tile_fov,
-                                int n_tiles) {
-    int f = blockIdx.x;
-    if (f >= nfov) return;
-    const smx_fov_t F = fovs[f];
-    int nt = F.tiles_r * F.tiles_c;
-    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-        int g = F.tile_off + t;
-        if (g >= 0 && g < n_tiles) tile_fov[g] = f;
-    }
-}
-
 struct SplatWorkspace {
     EmitterRec* rec;   // n (phase-table path)
     int* tile_count;   // n_tiles + 1
@@ -778,8 +744,6 @@ struct SplatWorkspace {
     void* cub_tmp;
     size_t cub_bytes;
 };
-
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t scan_tmp_bytes(int n_tiles) {
     size_t bytes = 0;

@@ -1,0 +1,197 @@
+"""Differentiable Gaussian model on the GPU against the float64 oracle:
+forward image, Poisson NLL, hand-written backward, float64 central finite
+differences, autograd plumbing, and an Adam refinement of positions."""
+import numpy
+import pytest
+import torch
+
+from oracle import gauss as G
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(sigma=1.2, inv_depth=1.0 / 0.377, background=5.0, radius=6)
+
+
+def _case(seed, shapes, n_per, shift=(0.25, -0.15)):
+    rng = numpy.random.RandomState(seed)
+    xs, ys, fid = [], [], []
+    for f, (nw, nh) in enumerate(shapes):
+        xs.append(rng.uniform(-2.0, nw + 2.0, n_per))
+        ys.append(rng.uniform(-2.0, nh + 2.0, n_per))
+        fid.append(numpy.full(n_per, f))
+    x, y, fid = numpy.concatenate(xs), numpy.concatenate(ys), numpy.concatenate(fid)
+    n = len(x)
+    z = rng.uniform(-0.5, 0.5, n)
+    I = rng.uniform(300.0, 900.0, n)
+    truth = G.render(shapes, x + shift[0], y + shift[1], z, I, fid, **KW)
+    k = [rng.poisson(m).astype(float) for m in truth]
+    return x, y, z, I, fid, k
+
+
+def _dev(a, dtype=torch.float32):
+    return torch.as_tensor(numpy.ascontiguousarray(a), dtype=dtype, device='cuda')
+
+
+def _model(shapes):
+    from scopyon_amd.gauss import GaussianModel
+    return GaussianModel(shapes, **KW)
+
+
+def test_forward_and_nll_match_oracle(gpu):
+    shapes = [(64, 64), (40, 72), (100, 33)]
+    x, y, z, I, fid, k = _case(0, shapes, 60)
+    m = _model(shapes)
+    mu = m.forward(_dev(x), _dev(y), _dev(z), _dev(I), _dev(fid, torch.int32))
+    assert m.check_status() & ~1 == 0
+    ref = G.render(shapes, x, y, z, I, fid, **KW)
+    for f in range(len(shapes)):
+        got = m.frame(mu, f).cpu().numpy().astype(float)
+        numpy.testing.assert_allclose(got, ref[f], rtol=2e-6, atol=1e-6 * ref[f].max())
+    kd = torch.cat([_dev(a.ravel()) for a in k])
+    grad = torch.empty_like(mu)
+    loss = m.nll(mu, kd, grad).cpu().numpy()
+    want = G.poisson_nll([m.frame(mu, f).cpu().numpy().astype(float) for f in range(3)],
+                         [a.astype(numpy.float32).astype(float) for a in k])
+    numpy.testing.assert_allclose(loss, want, rtol=1e-12)
+    mu_h = mu.cpu().numpy().astype(numpy.float32)
+    numpy.testing.assert_allclose(grad.cpu().numpy(), 1.0 - kd.cpu().numpy() / mu_h, rtol=1e-6)
+
+
+def test_forward_is_deterministic(gpu):
+    shapes = [(96, 96)]
+    x, y, z, I, fid, _ = _case(1, shapes, 400)
+    m = _model(shapes)
+    args = [_dev(x), _dev(y), _dev(z), _dev(I), None]
+    a = m.forward(*args).clone()
+    b = m.forward(*args).clone()
+    assert torch.equal(a, b)
+
+
+def _gpu_grads(m, x, y, z, I, fid, k):
+    mu = m.forward(_dev(x), _dev(y), _dev(z), _dev(I), _dev(fid, torch.int32))
+    kd = torch.cat([_dev(a.ravel()) for a in k])
+    grad = torch.empty_like(mu)
+    m.nll(mu, kd, grad)
+    gx, gy, gz, gI, gs = m.backward(_dev(x), _dev(y), _dev(z), _dev(I), grad,
+                                    _dev(fid, torch.int32))
+    return [t.cpu().numpy().astype(float) for t in (gx, gy, gz, gI)] + [float(gs.item())]
+
+
+def test_backward_matches_oracle_gradients(gpu):
+    shapes = [(48, 48), (32, 60)]
+    x, y, z, I, fid, k = _case(2, shapes, 40)
+    m = _model(shapes)
+    got = _gpu_grads(m, x, y, z, I, fid, k)
+    kf = [a.astype(numpy.float32).astype(float) for a in k]
+    want = G.gradients(shapes, x, y, z, I, fid, targets=kf, **KW)
+    for g, w, name in zip(got[:4], want[:4], 'x y z I'.split()):
+        scale = numpy.abs(w).max()
+        numpy.testing.assert_allclose(g, w, rtol=1e-4, atol=1e-4 * scale, err_msg=name)
+    assert abs(got[4] - want[4]) <= 1e-4 * abs(want[4]) + 1e-3
+
+
+def test_backward_against_float64_central_differences(gpu):
+    """GPU float32 gradients vs float64 central differences of the oracle
+    (h = 1e-4 px for positions, 1e-4 um for z, 1e-2 for I, 1e-4 px for sigma),
+    both at the float32 parameters the GPU sees.  Error bound: 1e-4 relative,
+    measured against max(|fd|, RMS of that parameter's gradients) -- a float32
+    image has a rounding floor of ~1e-7 per pixel, so components that nearly
+    cancel cannot be resolved to 1e-4 of their own size."""
+    shapes = [(40, 40)]
+    x, y, z, I, fid, k = _case(3, shapes, 25)
+    x, y, z, I = (a.astype(numpy.float32).astype(float) for a in (x, y, z, I))
+    sig32 = float(numpy.float32(KW['sigma']))
+    m = _model(shapes)
+    got = _gpu_grads(m, x, y, z, I, fid, k)
+    kf = [a.astype(numpy.float32).astype(float) for a in k]
+    base = dict(sigma=sig32, inv_depth=float(numpy.float32(KW['inv_depth'])),
+                background=KW['background'], radius=KW['radius'])
+
+    def L(xx, yy, zz, II, sig):
+        b = dict(base)
+        b['sigma'] = sig
+        return G.loss(shapes, xx, yy, zz, II, fid, targets=kf, **b)
+
+    for idx, h in enumerate((1e-4, 1e-4, 1e-4, 1e-2)):
+        fds, gs = [], []
+        for e in range(len(x)):
+            p, q = [a.copy() for a in (x, y, z, I)], [a.copy() for a in (x, y, z, I)]
+            p[idx][e] += h
+            q[idx][e] -= h
+            fds.append((L(*p, sig32) - L(*q, sig32)) / (2 * h))
+            gs.append(got[idx][e])
+        fds, gs = numpy.array(fds), numpy.array(gs)
+        rms = numpy.sqrt(numpy.mean(fds ** 2))
+        rel = numpy.abs(gs - fds) / numpy.maximum(numpy.abs(fds), rms)
+        assert rel.max() <= 1e-4, ('xyzI'[idx], rel.max(), int(rel.argmax()))
+        big = numpy.abs(fds) >= 0.1 * numpy.abs(fds).max()
+        assert (numpy.abs(gs - fds)[big] / numpy.abs(fds[big])).max() <= 1e-4
+    fd = (L(x, y, z, I, sig32 + 1e-4) - L(x, y, z, I, sig32 - 1e-4)) / 2e-4
+    assert abs(got[4] - fd) <= 1e-4 * abs(fd), ('sigma', got[4], fd)
+
+
+def test_autograd_function_and_adam_refinement(gpu):
+    from scopyon_amd.gauss import render, poisson_nll
+    shapes = [(64, 64), (64, 64)]
+    rng = numpy.random.RandomState(4)
+    n_per = 30
+    tx = rng.uniform(8, 56, 2 * n_per)
+    ty = rng.uniform(8, 56, 2 * n_per)
+    fid = numpy.repeat([0, 1], n_per)
+    z = rng.uniform(-0.3, 0.3, 2 * n_per)
+    I = numpy.full(2 * n_per, 2000.0)
+    truth = G.render(shapes, tx, ty, z, I, fid, **KW)
+    k = torch.cat([_dev(rng.poisson(t).astype(float).ravel()) for t in truth])
+    m = _model(shapes)
+    x = _dev(tx + rng.normal(0, 0.4, 2 * n_per)).requires_grad_(True)
+    y = _dev(ty + rng.normal(0, 0.4, 2 * n_per)).requires_grad_(True)
+    zt, It, f32 = _dev(z), _dev(I), _dev(fid, torch.int32)
+    # the autograd path equals the raw kernels
+    mu = render(m, x, y, zt, It, None, f32)
+    loss = poisson_nll(m, mu, k).sum()
+    loss.backward()
+    mu2 = m.forward(x.detach(), y.detach(), zt, It, f32)
+    g2 = torch.empty_like(mu2)
+    m.nll(mu2, k, g2)
+    gx, gy, _, _, _ = m.backward(x.detach(), y.detach(), zt, It, g2, f32)
+    assert torch.allclose(x.grad, gx) and torch.allclose(y.grad, gy)
+    err0 = float(numpy.sqrt(numpy.mean((x.detach().cpu().numpy() - tx) ** 2 +
+                                       (y.detach().cpu().numpy() - ty) ** 2)))
+    opt = torch.optim.Adam([x, y], lr=0.05)
+    first = None
+    for it in range(200):
+        opt.zero_grad()
+        loss = poisson_nll(m, render(m, x, y, zt, It, None, f32), k).sum()
+        loss.backward()
+        opt.step()
+        first = float(loss) if first is None else first
+    err = float(numpy.sqrt(numpy.mean((x.detach().cpu().numpy() - tx) ** 2 +
+                                      (y.detach().cpu().numpy() - ty) ** 2)))
+    assert float(loss) < first
+    assert err < 0.25 * err0, (err0, err)
+
+
+def test_sigma_gradient_through_autograd(gpu):
+    from scopyon_amd.gauss import render, poisson_nll
+    shapes = [(48, 48)]
+    x, y, z, I, fid, k = _case(5, shapes, 30)
+    m = _model(shapes)
+    sig = torch.tensor(KW['sigma'], dtype=torch.float32, device='cuda', requires_grad=True)
+    kd = _dev(k[0].ravel())
+    loss = poisson_nll(m, render(m, _dev(x), _dev(y), _dev(z), _dev(I), sig, None), kd).sum()
+    loss.backward()
+    kf = [k[0].astype(numpy.float32).astype(float)]
+    want = G.gradients(shapes, x, y, z, I, fid, targets=kf, **KW)[4]
+    assert abs(float(sig.grad) - want) <= 1e-4 * abs(want) + 1e-3
+
+
+def test_bad_arguments(gpu):
+    from scopyon_amd.gauss import GaussianModel
+    with pytest.raises(ValueError):
+        GaussianModel([(8, 8)], radius=9)
+    m = GaussianModel([(8, 8), (8, 8)])
+    with pytest.raises(ValueError):
+        m.forward(_dev([1.0]), _dev([1.0]), _dev([0.0]), _dev([1.0]))
+    with pytest.raises(TypeError):
+        m.forward(_dev([1.0], torch.float64), _dev([1.0]), _dev([0.0]), _dev([1.0]),
+                  _dev([0], torch.int32))
