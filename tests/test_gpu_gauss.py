@@ -164,10 +164,10 @@ def test_autograd_function_and_adam_refinement(gpu):
         loss = poisson_nll(m, render(m, x, y, zt, It, None, f32), k).sum()
         loss.backward()
         opt.step()
-        first = float(loss) if first is None else first
+        first = float(loss.detach()) if first is None else first
     err = float(numpy.sqrt(numpy.mean((x.detach().cpu().numpy() - tx) ** 2 +
                                       (y.detach().cpu().numpy() - ty) ** 2)))
-    assert float(loss) < first
+    assert float(loss.detach()) < first
     assert err < 0.25 * err0, (err0, err)
 
 
