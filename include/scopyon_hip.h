@@ -187,6 +187,18 @@ int smx_gauss_forward(int n, const float* x, const float* y, const float* z, con
                       const smx_gauss_params_t* params /* host */, long long capacity, float* mu,
                       void* workspace, size_t workspace_bytes, int* status_dev,
                       hipStream_t stream);
+/* The same image on the matrix cores, where the PSF is separable: per 32 x 32
+ * tile, mu = background + A B with A[i][e] = I_e a(z_e) G(i; x_e) and
+ * B[e][j] = G(j; y_e) over the tile's emitters (zero outside each window),
+ * the dense contraction Gx^T diag(I) Gy.  dtype 0: float32 operands
+ * (v_mfma_f32_32x32x2_f32), 1: float16, 2: bfloat16 (v_mfma_f32_32x32x16_*),
+ * float32 accumulation.  z may be NULL (no depth factor).  Same workspace and
+ * status bits as smx_gauss_forward. */
+int smx_sep_gemm(int n, const float* x, const float* y, const float* z, const float* I,
+                 const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                 const smx_gauss_params_t* params /* host */, int dtype, long long capacity,
+                 float* mu, void* workspace, size_t workspace_bytes, int* status_dev,
+                 hipStream_t stream);
 /* loss[f] = sum over frame f of mu - k log(mu) (float64); grad_mu (may be NULL)
  * = 1 - k / mu.  workspace: smx_gauss_nll_workspace_bytes(nframes, max pixels
  * of any frame). */

@@ -24,6 +24,7 @@
 #include "tiles.hpp"
 
 #include <hipcub/hipcub.hpp>
+#include <type_traits>
 
 namespace {
 
@@ -397,6 +398,165 @@ __global__ void __launch_bounds__(1024) sum_f64_kernel(const double* __restrict_
     if (threadIdx.x == 0) out[0] = s_red[0];
 }
 
+// --------------------------------------------------------------------------
+// Separable splat on the matrix cores.  Per 32 x 32 tile the image is the
+// contraction  mu_tile = A B,  A[i][e] = amp_e G(i; x_e),  B[e][j] = G(j; y_e)
+// over the tile's emitters e (zero outside each emitter's window): one
+// 32 x 32 MFMA accumulator per tile, emitters streamed 16 at a time in index
+// order.  Inputs are rounded to the operand type (float32, float16 or
+// bfloat16); accumulation is float32.
+// --------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Per emitter: the window origin and the (2R+1)-pixel factors along each axis,
+// from one tail mass per pixel edge, q(b) = erfc(|b - x| / (sqrt(2) sigma)) / 2:
+// G = q(i) - q(i+1) right of x, q(i+1) - q(i) left of x, 1 - q(i) - q(i+1)
+// across x.  32 lanes per emitter: lane t handles edge t % 16 of axis t / 16.
+__global__ void __launch_bounds__(256) sep_factor_kernel(int n, const float* __restrict__ x,
+                                                         const float* __restrict__ y,
+                                                         const float* __restrict__ z,
+                                                         const float* __restrict__ I,
+                                                         smx_gauss_params_t P,
+                                                         float* __restrict__ fac,
+                                                         int2* __restrict__ lo_out) {
+    const int e = blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int t = threadIdx.x & 31, ax = t >> 4, b = t & 15;
+    const int R = P.radius, W = 2 * R + 1;
+    const float s = INV_SQRT2 / P.sigma;
+    const bool valid = e < n;
+    const float pos = valid ? (ax ? y[e] : x[e]) : 0.0f;
+    const bool fin = finite_coord(pos);
+    const int lo = fin ? (int)floorf(pos) - R : 0;
+    const float tb = ((float)(lo + b) - pos) * s;
+    const float q = 0.5f * erfcf(fabsf(tb));
+    const float qn = __shfl_down(q, 1, 16);          // edge b + 1 (same axis)
+    const float edge = (float)(lo + b);
+    float g;
+    if (edge >= pos) g = q - qn;                      // pixel right of x
+    else if (edge + 1.0f <= pos) g = qn - q;          // pixel left of x
+    else g = 1.0f - q - qn;                           // pixel containing x
+    if (b >= W || !fin) g = 0.0f;
+    if (valid) {
+        float amp = 1.0f;
+        if (ax == 0) amp = I[e] * (z ? expf(-fabsf(z[e]) * P.inv_depth) : 1.0f);
+        fac[(size_t)e * 32 + t] = ax == 0 ? amp * g : g;
+        if (t == 0) {
+            const float xv = x[e], yv = y[e];
+            lo_out[e] = make_int2(finite_coord(xv) ? (int)floorf(xv) - R : 0,
+                                  finite_coord(yv) ? (int)floorf(yv) - R : 0);
+        }
+    }
+}
+
+struct SepArgs {
+    const float* fac;
+    const int2* lo;
+    const smx_fov_t* frames;
+    int nframes;
+    const int* tile_fov;
+    const int* tile_begin;
+    const int* list;
+    long long capacity;
+    int W;
+    float background;
+    float* mu;
+    int* status;
+};
+
+template <typename T> struct Frag8 { typedef T type __attribute__((ext_vector_type(8))); };
+
+template <int DT>   // 0: float32 (32x32x2), 1: float16 (32x32x16), 2: bfloat16 (32x32x16)
+__global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
+    __shared__ int s_sort[PSORT_CAP];
+    __shared__ float s_fac[16][33];
+    __shared__ int2 s_lo[16];
+    const int tile = blockIdx.x;
+    const int fr = A.tile_fov[tile];
+    if (fr < 0 || fr >= A.nframes) return;
+    const smx_fov_t F = A.frames[fr];
+    const int local = tile - F.tile_off;
+    const int tr = local / F.tiles_c, tc = local % F.tiles_c;
+    const int l = threadIdx.x;
+    const int m = l & 31, half = l >> 5;
+    const int row = tr * TS + m, col = tc * TS + m;   // A row / B column of this lane
+
+    const int begin = A.tile_begin[tile];
+    int count = A.tile_begin[tile + 1] - begin;
+    if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
+    const int* list = A.list + begin;
+    const bool sorted = count <= PSORT_CAP;
+    if (count >= 1 && sorted) {
+        int np2 = 1;
+        while (np2 < count) np2 <<= 1;
+        for (int k = l; k < np2; k += 64) s_sort[k] = k < count ? list[k] : 0x7fffffff;
+        __syncthreads();
+        if (count > 1) lds_bitonic_sort(s_sort, np2);
+    } else if (!sorted && l == 0) {
+        atomicOr(A.status, 1);
+    }
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    const int W = A.W;
+    for (int base = 0; base < count; base += 16) {
+        const int nk = min(16, count - base);
+        // stage the chunk's factors: 16 emitters x 32 floats, 8 per lane
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int idx = r * 64 + l, k = idx >> 5, t = idx & 31;
+            float v = 0.0f;
+            if (k < nk) {
+                const int e = sorted ? s_sort[base + k] : list[base + k];
+                v = A.fac[(size_t)e * 32 + t];
+                if (t == 0) s_lo[k] = A.lo[e];
+            }
+            s_fac[k][t] = v;
+        }
+        __syncthreads();
+        if (DT == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = 2 * j + half;
+                const int2 o = s_lo[k];
+                const int ru = row - o.x, cv = col - o.y;
+                const bool ok = k < nk;
+                const float a = (ok && (unsigned)ru < (unsigned)W) ? s_fac[k][ru] : 0.0f;
+                const float b = (ok && (unsigned)cv < (unsigned)W) ? s_fac[k][16 + cv] : 0.0f;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
+        } else {
+            typedef typename std::conditional<DT == 1, _Float16, __bf16>::type E;
+            typedef typename Frag8<E>::type V8;
+            V8 a, b;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int k = half * 8 + t;
+                const int2 o = s_lo[k];
+                const int ru = row - o.x, cv = col - o.y;
+                const bool ok = k < nk;
+                const float av = (ok && (unsigned)ru < (unsigned)W) ? s_fac[k][ru] : 0.0f;
+                const float bv = (ok && (unsigned)cv < (unsigned)W) ? s_fac[k][16 + cv] : 0.0f;
+                a[t] = (E)av;
+                b[t] = (E)bv;
+            }
+            if constexpr (DT == 1) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
+            else acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // acc[i] holds row 8 (i / 4) + 4 half + i % 4, column m of the tile
+    const int pj = tc * TS + m;
+    if (pj < F.nh) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int pi = tr * TS + 8 * (i >> 2) + 4 * half + (i & 3);
+            if (pi < F.nw) A.mu[(size_t)F.pix_off + (size_t)pi * F.nh + pj] = A.background + acc[i];
+        }
+    }
+}
+
 struct GaussWorkspace {
     int* tile_count;   // n_tiles + 1
     int* tile_begin;   // n_tiles + 1
@@ -404,6 +564,8 @@ struct GaussWorkspace {
     int* tile_fov;     // n_tiles
     int* list;         // capacity
     double* gsig;      // n
+    float* fac;        // n x 32: amp * G along rows (16), G along columns (16)
+    int2* lo;          // n: window origin
     void* cub_tmp;
     size_t cub_bytes;
 };
@@ -417,6 +579,8 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
     size_t o_sig = take(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    size_t o_fac = take(sizeof(float) * 32 * (size_t)(n > 0 ? n : 1));
+    size_t o_lo = take(sizeof(int2) * (size_t)(n > 0 ? n : 1));
     size_t cb = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cb, (int*)nullptr, (int*)nullptr, n_tiles + 1);
     size_t o_cub = take(cb);
@@ -427,6 +591,8 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
         ws->gsig = (double*)(base + o_sig);
+        ws->fac = (float*)(base + o_fac);
+        ws->lo = (int2*)(base + o_lo);
         ws->cub_tmp = base + o_cub;
         ws->cub_bytes = cb;
     }
@@ -449,26 +615,14 @@ size_t smx_gauss_workspace_bytes(int n, int n_tiles, long long capacity) {
     return gauss_layout(n, n_tiles, capacity, nullptr, nullptr);
 }
 
-int smx_gauss_forward(int n, const float* x, const float* y, const float* z, const float* I,
-                      const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
-                      const smx_gauss_params_t* params, long long capacity, float* mu,
-                      void* workspace, size_t workspace_bytes, int* status_dev,
-                      hipStream_t stream) {
-    if (n < 0 || nframes <= 0 || n_tiles <= 0 || !frames_dev || !mu || !workspace ||
-        !status_dev || !params_ok(params) || (n > 0 && (!x || !y || !z || !I))) {
-        smx::set_error("smx_gauss_forward: invalid arguments (n=%d, nframes=%d, n_tiles=%d)", n,
-                       nframes, n_tiles);
-        return SMX_EVALUE;
-    }
-    size_t need = gauss_layout(n, n_tiles, capacity, nullptr, nullptr);
-    if (workspace_bytes < need) {
-        smx::set_error("smx_gauss_forward: workspace too small (%zu < %zu bytes)", workspace_bytes,
-                       need);
-        return SMX_EVALUE;
-    }
-    GaussWorkspace ws;
-    gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
-    const int R = params->radius;
+}  // extern "C"
+
+namespace {
+
+// Tile binning shared by both forward forms: count -> scan -> fill.
+int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
+              const smx_fov_t* frames_dev, int nframes, int n_tiles, int R, long long capacity,
+              GaussWorkspace& ws, int* status_dev, hipStream_t stream) {
     const int nb = (n + 255) / 256;
     hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0, stream,
                        ws.tile_count, (size_t)n_tiles + 1, 0);
@@ -498,12 +652,85 @@ int smx_gauss_forward(int n, const float* x, const float* y, const float* z, con
                            ws.tile_cursor, ws.list, capacity, status_dev);
         SMX_CHECK_LAUNCH("gauss_bin_fill_kernel");
     }
+    return SMX_OK;
+}
+
+int forward_checks(const char* who, int n, const float* x, const float* y, const float* z,
+                   const float* I, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                   const smx_gauss_params_t* params, long long capacity, const float* mu,
+                   void* workspace, size_t workspace_bytes, const int* status_dev,
+                   bool z_required) {
+    if (n < 0 || nframes <= 0 || n_tiles <= 0 || !frames_dev || !mu || !workspace ||
+        !status_dev || !params_ok(params) || capacity < 0 ||
+        (n > 0 && (!x || !y || !I || (z_required && !z)))) {
+        smx::set_error("%s: invalid arguments (n=%d, nframes=%d, n_tiles=%d)", who, n, nframes,
+                       n_tiles);
+        return SMX_EVALUE;
+    }
+    size_t need = gauss_layout(n, n_tiles, capacity, nullptr, nullptr);
+    if (workspace_bytes < need) {
+        smx::set_error("%s: workspace too small (%zu < %zu bytes)", who, workspace_bytes, need);
+        return SMX_EVALUE;
+    }
+    return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smx_gauss_forward(int n, const float* x, const float* y, const float* z, const float* I,
+                      const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                      const smx_gauss_params_t* params, long long capacity, float* mu,
+                      void* workspace, size_t workspace_bytes, int* status_dev,
+                      hipStream_t stream) {
+    int rc = forward_checks("smx_gauss_forward", n, x, y, z, I, frames_dev, nframes, n_tiles,
+                            params, capacity, mu, workspace, workspace_bytes, status_dev, true);
+    if (rc) return rc;
+    GaussWorkspace ws;
+    gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
+    rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, params->radius, capacity, ws,
+                   status_dev, stream);
+    if (rc) return rc;
     FwdArgs A;
     A.x = x; A.y = y; A.z = z; A.I = I; A.frames = frames_dev; A.nframes = nframes;
     A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
     A.capacity = capacity; A.P = *params; A.mu = mu; A.status = status_dev;
     hipLaunchKernelGGL(gauss_forward_kernel, dim3(n_tiles), dim3(256), 0, stream, A);
     SMX_CHECK_LAUNCH("gauss_forward_kernel");
+    return SMX_OK;
+}
+
+int smx_sep_gemm(int n, const float* x, const float* y, const float* z, const float* I,
+                 const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                 const smx_gauss_params_t* params, int dtype, long long capacity, float* mu,
+                 void* workspace, size_t workspace_bytes, int* status_dev, hipStream_t stream) {
+    int rc = forward_checks("smx_sep_gemm", n, x, y, z, I, frames_dev, nframes, n_tiles, params,
+                            capacity, mu, workspace, workspace_bytes, status_dev, false);
+    if (rc) return rc;
+    if (dtype < 0 || dtype > 2) {
+        smx::set_error("smx_sep_gemm: dtype must be 0 (float32), 1 (float16) or 2 (bfloat16)");
+        return SMX_EVALUE;
+    }
+    GaussWorkspace ws;
+    gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
+    rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, params->radius, capacity, ws,
+                   status_dev, stream);
+    if (rc) return rc;
+    if (n > 0) {
+        hipLaunchKernelGGL(sep_factor_kernel, dim3((n + 7) / 8), dim3(256), 0, stream, n, x, y, z,
+                           I, *params, ws.fac, ws.lo);
+        SMX_CHECK_LAUNCH("sep_factor_kernel");
+    }
+    SepArgs A;
+    A.fac = ws.fac; A.lo = ws.lo; A.frames = frames_dev; A.nframes = nframes;
+    A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
+    A.capacity = capacity; A.W = 2 * params->radius + 1; A.background = params->background;
+    A.mu = mu; A.status = status_dev;
+    if (dtype == 0) hipLaunchKernelGGL(sep_gemm_kernel<0>, dim3(n_tiles), dim3(64), 0, stream, A);
+    else if (dtype == 1) hipLaunchKernelGGL(sep_gemm_kernel<1>, dim3(n_tiles), dim3(64), 0, stream, A);
+    else hipLaunchKernelGGL(sep_gemm_kernel<2>, dim3(n_tiles), dim3(64), 0, stream, A);
+    SMX_CHECK_LAUNCH("sep_gemm_kernel");
     return SMX_OK;
 }
 

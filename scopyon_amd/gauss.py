@@ -32,8 +32,11 @@ class GaussianModel:
     frames of possibly different sizes."""
 
     def __init__(self, frame_shapes, *, sigma=1.2, inv_depth=1.0 / 0.377, background=10.0,
-                 radius=6, device=None):
+                 radius=6, method='lds', device=None):
         self.device = device or _device.require_gpu()
+        if method not in self.METHODS:
+            raise ValueError('method must be one of {}'.format(sorted(self.METHODS)))
+        self.method = method
         if not 1 <= int(radius) <= 7:
             raise ValueError('radius must be in 1..7 (window of at most 15 pixels)')
         if not sigma > 0:
@@ -91,7 +94,7 @@ class GaussianModel:
 
     @staticmethod
     def _check(t, name):
-        if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+        if t is None or t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
             raise TypeError('{} must be a contiguous float32 CUDA tensor'.format(name))
 
     def _frame_ids(self, frame_id, n):
@@ -104,19 +107,36 @@ class GaussianModel:
         return frame_id
 
     # ---- raw kernels (no autograd) ----
-    def forward(self, x, y, z, I, frame_id=None, sigma=None, out=None):
-        for t, nm in ((x, 'x'), (y, 'y'), (z, 'z'), (I, 'I')):
+    METHODS = {'lds': None, 'mfma-f32': 0, 'mfma-f16': 1, 'mfma-bf16': 2}
+
+    def forward(self, x, y, z, I, frame_id=None, sigma=None, out=None, method=None):
+        """Expected image.  method: 'lds' (float32 products added in LDS, the
+        default) or 'mfma-f32' / 'mfma-f16' / 'mfma-bf16' (the separable
+        contraction on the matrix cores with float32, float16 or bfloat16
+        operands and float32 accumulation).  z may be None with the MFMA forms."""
+        method = method or self.method
+        if method not in self.METHODS:
+            raise ValueError('method must be one of {}'.format(sorted(self.METHODS)))
+        for t, nm in ((x, 'x'), (y, 'y'), (I, 'I')):
             self._check(t, nm)
+        if z is not None or method == 'lds':
+            self._check(z, 'z')
         n = x.numel()
         fid = self._frame_ids(frame_id, n)
         ws, cap = self._workspace(n)
         mu = out if out is not None else torch.empty(self.n_pix, dtype=torch.float32,
                                                      device=self.device)
         p = self.params(sigma)
-        _lib.call('smx_gauss_forward', n, x.data_ptr(), y.data_ptr(), z.data_ptr(), I.data_ptr(),
-                  _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes, self.n_tiles,
-                  ctypes.byref(p), cap, mu.data_ptr(), ws.data_ptr(), ws.numel(),
-                  self.status.data_ptr(), _device.stream())
+        if method == 'lds':
+            _lib.call('smx_gauss_forward', n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                      I.data_ptr(), _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes,
+                      self.n_tiles, ctypes.byref(p), cap, mu.data_ptr(), ws.data_ptr(),
+                      ws.numel(), self.status.data_ptr(), _device.stream())
+        else:
+            _lib.call('smx_sep_gemm', n, x.data_ptr(), y.data_ptr(), _device.ptr(z), I.data_ptr(),
+                      _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes, self.n_tiles,
+                      ctypes.byref(p), self.METHODS[method], cap, mu.data_ptr(), ws.data_ptr(),
+                      ws.numel(), self.status.data_ptr(), _device.stream())
         return mu
 
     def nll(self, mu, k, grad_mu=None):

@@ -195,3 +195,60 @@ def test_bad_arguments(gpu):
     with pytest.raises(TypeError):
         m.forward(_dev([1.0], torch.float64), _dev([1.0]), _dev([0.0]), _dev([1.0]),
                   _dev([0], torch.int32))
+
+
+@pytest.mark.parametrize('method,rms_tol,max_tol', [('mfma-f32', 2e-6, 2e-5),
+                                                    ('mfma-f16', 1e-3, 5e-3),
+                                                    ('mfma-bf16', 8e-3, 4e-2),
+                                                    ('lds', 2e-6, 2e-5)])
+def test_separable_forms_against_oracle(gpu, method, rms_tol, max_tol):
+    """Matrix-core contraction (float32/float16/bfloat16 operands, float32
+    accumulation) and the LDS splat vs the float64 oracle: relative RMS error
+    ||mu - ref|| / ||ref|| and the largest relative error over pixels above 1%
+    of the frame maximum (intensities exponential, mean 500; no background)."""
+    from scopyon_amd.gauss import GaussianModel
+    shapes = [(128, 96), (64, 64)]
+    rng = numpy.random.RandomState(6)
+    n_per = 300
+    x = numpy.concatenate([rng.uniform(-3, s[0] + 3, n_per) for s in shapes])
+    y = numpy.concatenate([rng.uniform(-3, s[1] + 3, n_per) for s in shapes])
+    fid = numpy.repeat(numpy.arange(len(shapes)), n_per)
+    I = rng.exponential(500.0, len(x))
+    z = numpy.zeros(len(x))
+    kw = dict(sigma=1.2, inv_depth=0.0, background=0.0, radius=6)
+    m = GaussianModel(shapes, method=method, **kw)
+    zt = None if method != 'lds' else _dev(z)
+    mu = m.forward(_dev(x), _dev(y), zt, _dev(I), _dev(fid, torch.int32))
+    assert m.check_status() & ~1 == 0
+    ref = G.render(shapes, x, y, z, I, fid, **kw)
+    for f in range(len(shapes)):
+        got = m.frame(mu, f).cpu().numpy().astype(float)
+        rms = numpy.linalg.norm(got - ref[f]) / numpy.linalg.norm(ref[f])
+        big = ref[f] > 0.01 * ref[f].max()
+        mx = (numpy.abs(got - ref[f])[big] / ref[f][big]).max()
+        assert rms <= rms_tol and mx <= max_tol, (method, f, rms, mx)
+
+
+def test_mfma_forward_deterministic_and_unsorted_lists(gpu):
+    """Repeatable bits for sorted tile lists; a tile list longer than the LDS
+    sort keeps the image (summation order aside) and reports status bit 1."""
+    from scopyon_amd.gauss import GaussianModel
+    shapes = [(64, 64)]
+    rng = numpy.random.RandomState(7)
+    kw = dict(sigma=1.2, inv_depth=0.0, background=1.0, radius=6)
+    x, y = rng.uniform(0, 64, 600), rng.uniform(0, 64, 600)
+    I = rng.uniform(1.0, 2.0, len(x))
+    m = GaussianModel(shapes, method='mfma-f32', **kw)
+    args = (_dev(x), _dev(y), None, _dev(I))
+    a = m.forward(*args).clone()
+    b = m.forward(*args).clone()
+    assert torch.equal(a, b)
+    assert m.check_status() == 0
+    x = numpy.concatenate([rng.uniform(8, 24, 2500), x])
+    y = numpy.concatenate([rng.uniform(8, 24, 2500), y])
+    I = rng.uniform(1.0, 2.0, len(x))
+    m = GaussianModel(shapes, method='mfma-f32', **kw)
+    a = m.forward(_dev(x), _dev(y), None, _dev(I))
+    assert m.check_status() & 1
+    ref = G.render(shapes, x, y, numpy.zeros(len(x)), I, numpy.zeros(len(x), int), **kw)[0]
+    numpy.testing.assert_allclose(a.view(64, 64).cpu().numpy(), ref, rtol=2e-5)
