@@ -161,7 +161,8 @@ int smx_emccd_parity(int n, const double* E, const double* noise, const double* 
                      const double* T_common, double* adc_out, double* signal_out,
                      int* status_dev, hipStream_t stream);
 /* EMCCD with Philox4x32-10 randomness: Poisson(E) -> Gamma(n, emgain) signal,
- * Gaussian readout noise, ADC.  Fuses _epifm.py:1436-1469.  The Philox frame
+ * Gaussian readout noise, ADC.  Fuses _epifm.py:1436-1469 in float32
+ * arithmetic (a statistical path; the parity path is smx_emccd_parity).  The Philox frame
  * counter is `frame` plus *frame_dev when frame_dev is not NULL. */
 int smx_emccd_philox(int n, const double* photons, const double* gain_map,
                      const double* offset_map, const smx_detector_params_t* params /* host */,

@@ -149,6 +149,8 @@ __global__ void __launch_bounds__(256) emccd_parity_kernel(
 // the sampler needs no stream state and stays inlined (no call spills).
 // ---------------------------------------------------------------------------
 __device__ inline float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }   // [0, 1)
+// natural log on the hardware log2 (v_log_f32, ~1e-7 relative)
+__device__ inline float ln_hw(float v) { return 0.69314718055994531f * __builtin_amdgcn_logf(v); }
 __device__ inline float u01f_open0(uint32_t w) { return 1.0f - u01f(w); }          // (0, 1]
 
 struct PixelRng {
@@ -158,11 +160,13 @@ struct PixelRng {
     }
 };
 
+// Box-Muller on the hardware transcendentals: v_log_f32 is log2, v_cos_f32
+// takes its argument in revolutions (cos(2 pi v)); both ~1e-7 accurate on
+// these ranges.
 __device__ inline float normal_f(uint32_t a, uint32_t b) {
-    float r = sqrtf(-2.0f * logf(u01f_open0(a)));
-    float s, c;
-    sincospif(2.0f * u01f(b), &s, &c);
-    return r * c;
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f *    // -2 ln 2
+                                           __builtin_amdgcn_logf(u01f_open0(a)));
+    return r * __builtin_amdgcn_cosf(u01f(b));
 }
 
 // Poisson for lam >= 12: Hormann's PTRS (transformed rejection with squeeze,
@@ -188,7 +192,7 @@ __device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
     return (int)floorf(lam + 0.5f);
 }
 
-// Gamma(n, 1), n >= 2: Marsaglia-Tsang (ACM TOMS 26(3), 2000), one Philox
+// Gamma(n, 1), n >= 1: Marsaglia-Tsang (ACM TOMS 26(3), 2000), one Philox
 // block per iteration (normal from two words, uniform from the third).
 __device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk) {
     const float d = (float)n - 1.0f / 3.0f;
@@ -201,7 +205,7 @@ __device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk) {
         v = v * v * v;
         float u = u01f_open0(r.z);
         if (u < 1.0f - 0.0331f * (x * x) * (x * x)) return d * v;
-        if (logf(u) < 0.5f * x * x + d * (1.0f - v + logf(v))) return d * v;
+        if (ln_hw(u) < 0.5f * x * x + d * (1.0f - v + ln_hw(v))) return d * v;
     }
     return d;
 }
@@ -216,21 +220,25 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (frame_dev) frame += *frame_dev;
-    const double E = expectation(photons[i], P);
+    // float32 expectation, noise and ADC (a statistical path, not a parity path)
+    float ph = (float)photons[i];
+    if (P.background_switch) ph += (float)P.background;
+    const float E = (float)P.qe * ph;
     PixelRng R{pix_base + (uint32_t)i, frame, (uint32_t)seed, (uint32_t)(seed >> 32)};
     const smx::u32x4 r0 = R.block(0);
-    double noise = 0.0;
-    if (P.readout_noise > 0.0) noise = P.readout_noise * (double)normal_f(r0.x, r0.y);
-    double S = 0.0;
-    if (E > 0.0) {
-        const float lam = (float)E;
+    float noise = 0.0f;
+    if (P.readout_noise > 0.0) noise = (float)P.readout_noise * normal_f(r0.x, r0.y);
+    float S = 0.0f;
+    if (E > 0.0f) {
+        const float lam = E;
         int k = 0;
         if (lam < 12.0f) {
+            // inversion: smallest k with u < P(X <= k)
             float u = u01f(r0.z);
-            float p = expf(-lam), cum = p;
+            float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * lam), cum = p;
             while (u > cum && k < 4096) {
                 ++k;
-                p *= lam / (float)k;
+                p *= lam * __builtin_amdgcn_rcpf((float)k);
                 cum += p;
             }
         } else {
@@ -238,28 +246,27 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
         }
         if (k > 0) {
             // EM register: S = round(Gamma(k, g)), redrawn below 0.5 (S >= 1)
+            // Marsaglia-Tsang for every shape k >= 1 (one code path for the
+            // whole wave: no divergence between k = 1 and k > 1)
             uint32_t blk = 1u << 16;
-            uint32_t first = r0.w;
-            double s = 0.0;
+            float s = 0.0f;
             for (int guard = 0; guard < 64; ++guard) {
-                float gam;
-                if (k == 1) {
-                    gam = -logf(u01f_open0(first));
-                } else {
-                    gam = gamma_mt(R, k, blk);
-                }
-                s = (double)gam * (double)P.emgain;
-                if (s >= 0.5) break;
-                first = R.block(blk++).x;
+                s = gamma_mt(R, k, blk) * (float)P.emgain;
+                if (s >= 0.5f) break;
             }
-            S = floor(s + 0.5);
-            if (S < 1.0) S = 1.0;
+            S = floorf(s + 0.5f);
+            if (S < 1.0f) S = 1.0f;
         }
     }
-    const double gain = gain_map ? gain_map[i] : P.gain;
-    const double offset = offset_map ? offset_map[i] : P.offset;
-    if (E_out) E_out[i] = E;
-    adc_out[i] = (OutT)adc_counts(S + noise, P, gain, offset);
+    // ADC (_epifm.py:1475-1484): clip at full well, / gain, + offset, clip
+    const float gain = (float)(gain_map ? gain_map[i] : P.gain);
+    const float offset = (float)(offset_map ? offset_map[i] : P.offset);
+    float x = S + noise;
+    x = fminf(x, (float)P.fullwell);
+    x = x * __builtin_amdgcn_rcpf(gain) + offset;
+    x = fminf(fmaxf(x, 0.0f), (float)P.adc_max);
+    if (E_out) E_out[i] = (double)E;
+    adc_out[i] = (OutT)x;
 }
 
 }  // namespace

@@ -124,3 +124,24 @@ def test_emccd_philox_moments(gpu):
         assert abs(S.var() / (2 * g * g * Ev) - 1) < 0.08
         assert abs((S == 0).mean() - numpy.exp(-Ev)) < 5 * numpy.sqrt(numpy.exp(-Ev) / n) + 1e-4
         assert (S == numpy.floor(S)).all()
+
+
+def test_emccd_philox_readout_noise_is_gaussian(gpu):
+    """Readout noise of the Philox path (Box-Muller on the hardware log2 and
+    cos): N(0, sigma^2) by mean, variance and a Kolmogorov-Smirnov test."""
+    from scipy import stats
+    from scopyon_amd import _lib, _device
+    p = oracle_params({'detector': {'image_size': [512, 512]}})
+    det = _det_params(p)
+    det.background_switch = 0
+    det.readout_noise = 100.0
+    det.gain, det.offset, det.fullwell, det.adc_max = 1.0, 1000.0, 1e300, 1e300
+    n = 512 * 512
+    photons = torch.zeros(n, dtype=torch.float64, device=gpu)
+    adc = torch.empty(n, dtype=torch.float32, device=gpu)
+    _lib.call('smx_emccd_philox', n, photons.data_ptr(), None, None, det, 99, 3, 0, None, None,
+              adc.data_ptr(), 1, _device.stream())
+    x = (adc.cpu().numpy().astype(float) - 1000.0) / 100.0
+    assert abs(x.mean()) < 5 / numpy.sqrt(n)
+    assert abs(x.var() - 1.0) < 5 * numpy.sqrt(2.0 / n)
+    assert stats.kstest(x, 'norm').pvalue > 1e-3
