@@ -170,6 +170,27 @@ int smx_emccd_philox(int n, const double* photons, const double* gain_map,
                      const uint32_t* frame_dev, double* E_out, void* adc_out,
                      int adc_is_float32, hipStream_t stream);
 
+/* CCD and CMOS with caller-drawn numbers (_epifm.py:329-353, 418-433,
+ * 1443-1469): adc = ADC(signal + noise) with signal the caller's
+ * RandomState.poisson photo-electrons and noise either noise_in[i] (CCD
+ * normal draws; NULL: none) or, when table_n > 0, the CMOS readout value
+ * table_values[searchsorted(table_cdf, noise_in[i], 'right')] -- the draw of
+ * RandomState.choice(values, p) given its uniform noise_in[i]. */
+int smx_adc_parity(int n, const double* signal, const double* noise_in, const double* table_values,
+                   const double* table_cdf, int table_n, const double* gain_map,
+                   const double* offset_map, const smx_detector_params_t* params /* host */,
+                   double* adc_out, hipStream_t stream);
+/* CCD (table_n == 0: Gaussian readout noise of params->readout_noise) or CMOS
+ * (readout noise drawn from the value table) with Philox4x32-10 randomness:
+ * Poisson(E) photo-electrons + noise, ADC; float32 arithmetic. */
+int smx_photon_detector_philox(int n, const double* photons, const double* gain_map,
+                               const double* offset_map,
+                               const smx_detector_params_t* params /* host */,
+                               const double* table_values, const double* table_cdf, int table_n,
+                               uint64_t seed, uint32_t frame, uint32_t pix_base,
+                               const uint32_t* frame_dev, double* E_out, void* adc_out,
+                               int adc_is_float32, hipStream_t stream);
+
 /* ---- separable Gaussian model: forward, Poisson NLL, backward ----------- */
 /* Frames reuse the smx_fov_t fields nw, nh, tiles_r, tiles_c, pix_off and
  * tile_off (the length fields are unused: coordinates are pixels).  Emitter e belongs to

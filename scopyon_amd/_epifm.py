@@ -15,6 +15,7 @@ Random numbers follow the caller's generator:
     host draws, statistically equivalent.
 """
 import copy
+import os
 import math
 import warnings
 
@@ -399,16 +400,15 @@ class _EPIFMSimulator:
     def detector(self, camera, rng, full_output):
         c = self.configs
         n = camera.numel()
-        if c.detector_type != 'EMCCD':
-            if c.detector_type in ('CMOS', 'CCD'):
-                raise NotImplementedError(
-                    'detector type {} is not implemented by scopyon_amd'.format(c.detector_type))
+        if c.detector_type not in ('EMCCD', 'CCD', 'CMOS'):
             raise RuntimeError("Unknown detector type was given [{}]. ".format(c.detector_type)
                                + "Use either one of 'CMOS', 'CCD' or 'EMCCD'.")
         st = _device.stream()
         E = torch.empty(n, dtype=torch.float64, device=self.device)
         adc = torch.empty(n, dtype=torch.float64, device=self.device)
         gain_map, offset_map = (None, None) if self._maps is None else self._maps
+        if c.detector_type != 'EMCCD':
+            return self._photon_detector(camera, rng, E, adc, gain_map, offset_map)
         if isinstance(rng, PhiloxRNG):
             frame = self._philox_frame
             self._philox_frame += 1
@@ -444,6 +444,54 @@ class _EPIFMSimulator:
                   uidx.data_ptr(), _device.ptr(gain_map), _device.ptr(offset_map), self.det,
                   E_common, T_common.data_ptr(), adc.data_ptr(), None,
                   self.splatter.status.data_ptr(), st)
+        return E, adc
+
+    def _cmos_table(self):
+        """CMOS readout-noise values and the cdf RandomState.choice builds from
+        their weights (_epifm.py:331-345): p = w / sum(w); cdf = cumsum(p) / cdf[-1]."""
+        if getattr(self, '_cmos', None) is None:
+            table = numpy.loadtxt(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                               'catalog', 'detector', 'RNDist_F40.csv'),
+                                  delimiter=',')
+            p = table[:, 1] / table[:, 1].sum()
+            cdf = p.cumsum()
+            cdf /= cdf[-1]
+            self._cmos = (_device.upload(table[:, 0], device=self.device),
+                          _device.upload(cdf, device=self.device), len(cdf))
+        return self._cmos
+
+    def _photon_detector(self, camera, rng, E, adc, gain_map, offset_map):
+        """CCD and CMOS (_epifm.py:329-353, 418-433, 1443-1469): Poisson
+        photo-electrons plus Gaussian (CCD) or tabulated (CMOS) readout noise."""
+        c = self.configs
+        n = camera.numel()
+        st = _device.stream()
+        cmos = c.detector_type == 'CMOS'
+        tv, tc, tn = self._cmos_table() if cmos else (None, None, 0)
+        if isinstance(rng, PhiloxRNG):
+            frame = self._philox_frame
+            self._philox_frame += 1
+            _lib.call('smx_photon_detector_philox', n, camera.data_ptr(), _device.ptr(gain_map),
+                      _device.ptr(offset_map), self.det, _device.ptr(tv), _device.ptr(tc), tn,
+                      rng.seed, frame, 0, None, E.data_ptr(), adc.data_ptr(), 0, st)
+            return E, adc
+        _lib.call('smx_detector_expectation', n, camera.data_ptr(), self.det, E.data_ptr(), None,
+                  st)
+        shape = c.detector_image_size
+        # reference order: the noise (CMOS: one RandomState.choice uniform per
+        # pixel; CCD: one vector normal draw), then rng.poisson per pixel
+        if cmos:
+            noise = _device.upload(rng.random_sample(n), device=self.device)
+        elif c.detector_readout_noise > 0:
+            noise = _device.upload(rng.normal(0, c.detector_readout_noise, shape),
+                                   device=self.device)
+        else:
+            noise = None
+        signal = rng.poisson(E.cpu().numpy().reshape(shape)).astype(numpy.float64)
+        sig = _device.upload(signal, device=self.device)
+        _lib.call('smx_adc_parity', n, sig.data_ptr(), _device.ptr(noise), _device.ptr(tv),
+                  _device.ptr(tc), tn, _device.ptr(gain_map), _device.ptr(offset_map), self.det,
+                  adc.data_ptr(), st)
         return E, adc
 
     # ---- frames ----

@@ -98,6 +98,11 @@ _lib = None
 
 
 SIGNATURES.update({
+    'smx_adc_parity': (c_int, [c_int, P, P, P, P, c_int, P, P, ctypes.POINTER(DetectorParamsT),
+                               P, P]),
+    'smx_photon_detector_philox': (c_int, [c_int, P, P, P, ctypes.POINTER(DetectorParamsT), P, P,
+                                           c_int, c_uint64, c_uint32, c_uint32, P, P, P, c_int,
+                                           P]),
     'smx_gauss_workspace_bytes': (c_size_t, [c_int, c_int, c_longlong]),
     'smx_gauss_forward': (c_int, [c_int, P, P, P, P, P, P, c_int, c_int,
                                   ctypes.POINTER(GaussParamsT), c_longlong, P, P, c_size_t, P, P]),

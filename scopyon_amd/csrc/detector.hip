@@ -269,6 +269,81 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
     adc_out[i] = (OutT)x;
 }
 
+// searchsorted(cdf, u, side='right') clamped to the last index: the draw of
+// RandomState.choice(values, p=p) given its uniform u (cdf as numpy builds it).
+__device__ inline int table_index(const double* __restrict__ cdf, int m, double u) {
+    int lo = 0, hi = m;   // first index with cdf[idx] > u
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < m ? lo : m - 1;
+}
+
+// CCD / CMOS with caller-drawn numbers (_epifm.py:1443-1469).
+__global__ void adc_parity_kernel(int n, const double* __restrict__ signal,
+                                  const double* __restrict__ noise_in,
+                                  const double* __restrict__ tvals,
+                                  const double* __restrict__ tcdf, int tn,
+                                  const double* __restrict__ gain_map,
+                                  const double* __restrict__ offset_map, smx_detector_params_t P,
+                                  double* __restrict__ adc_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double noise = 0.0;
+    if (noise_in) noise = tn > 0 ? tvals[table_index(tcdf, tn, noise_in[i])] : noise_in[i];
+    const double gain = gain_map ? gain_map[i] : P.gain;
+    const double offset = offset_map ? offset_map[i] : P.offset;
+    adc_out[i] = adc_counts(signal[i] + noise, P, gain, offset);
+}
+
+// CCD (Gaussian readout noise) or CMOS (readout noise from a value table)
+// with Philox randomness: Poisson(E) photo-electrons + noise, float32 ADC.
+template <typename OutT>
+__global__ void photon_philox_kernel(int n, const double* __restrict__ photons,
+                                     const double* __restrict__ gain_map,
+                                     const double* __restrict__ offset_map,
+                                     smx_detector_params_t P, const double* __restrict__ tvals,
+                                     const double* __restrict__ tcdf, int tn, uint64_t seed,
+                                     uint32_t frame, uint32_t pix_base,
+                                     const uint32_t* __restrict__ frame_dev,
+                                     double* __restrict__ E_out, OutT* __restrict__ adc_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (frame_dev) frame += *frame_dev;
+    float ph = (float)photons[i];
+    if (P.background_switch) ph += (float)P.background;
+    const float E = (float)P.qe * ph;
+    PixelRng R{pix_base + (uint32_t)i, frame, (uint32_t)seed, (uint32_t)(seed >> 32)};
+    const smx::u32x4 r0 = R.block(0);
+    float noise = 0.0f;
+    if (tn > 0) noise = (float)tvals[table_index(tcdf, tn, smx::u01_53(r0.x, r0.y))];
+    else if (P.readout_noise > 0.0) noise = (float)P.readout_noise * normal_f(r0.x, r0.y);
+    int k = 0;
+    if (E > 0.0f) {
+        if (E < 12.0f) {
+            float u = u01f(r0.z);
+            float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * E), cum = p;
+            while (u > cum && k < 4096) {
+                ++k;
+                p *= E * __builtin_amdgcn_rcpf((float)k);
+                cum += p;
+            }
+        } else {
+            k = poisson_ptrs(R, E);
+        }
+    }
+    const float gain = (float)(gain_map ? gain_map[i] : P.gain);
+    const float offset = (float)(offset_map ? offset_map[i] : P.offset);
+    float x = (float)k + noise;
+    x = fminf(x, (float)P.fullwell);
+    x = x * __builtin_amdgcn_rcpf(gain) + offset;
+    x = fminf(fmaxf(x, 0.0f), (float)P.adc_max);
+    if (E_out) E_out[i] = (double)E;
+    adc_out[i] = (OutT)x;
+}
+
 }  // namespace
 
 extern "C" {
@@ -357,6 +432,49 @@ int smx_emccd_philox(int n, const double* photons, const double* gain_map,
                            stream, n, photons, gain_map, offset_map, *params, seed, frame,
                            pix_base, frame_dev, E_out, (double*)adc_out);
     SMX_CHECK_LAUNCH("emccd_philox_kernel");
+    return SMX_OK;
+}
+
+int smx_adc_parity(int n, const double* signal, const double* noise_in, const double* table_values,
+                   const double* table_cdf, int table_n, const double* gain_map,
+                   const double* offset_map, const smx_detector_params_t* params, double* adc_out,
+                   hipStream_t stream) {
+    if (n < 0 || !params || table_n < 0 || (table_n > 0 && (!table_values || !table_cdf)) ||
+        (n > 0 && (!signal || !adc_out))) {
+        smx::set_error("smx_adc_parity: invalid arguments (n=%d, table_n=%d)", n, table_n);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    hipLaunchKernelGGL(adc_parity_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, signal,
+                       noise_in, table_values, table_cdf, table_n, gain_map, offset_map, *params,
+                       adc_out);
+    SMX_CHECK_LAUNCH("adc_parity_kernel");
+    return SMX_OK;
+}
+
+int smx_photon_detector_philox(int n, const double* photons, const double* gain_map,
+                               const double* offset_map, const smx_detector_params_t* params,
+                               const double* table_values, const double* table_cdf, int table_n,
+                               uint64_t seed, uint32_t frame, uint32_t pix_base,
+                               const uint32_t* frame_dev, double* E_out, void* adc_out,
+                               int adc_is_float32, hipStream_t stream) {
+    if (n < 0 || !params || table_n < 0 || (table_n > 0 && (!table_values || !table_cdf)) ||
+        (n > 0 && (!photons || !adc_out))) {
+        smx::set_error("smx_photon_detector_philox: invalid arguments (n=%d)", n);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    if (adc_is_float32)
+        hipLaunchKernelGGL(photon_philox_kernel<float>, dim3((n + 255) / 256), dim3(256), 0,
+                           stream, n, photons, gain_map, offset_map, *params, table_values,
+                           table_cdf, table_n, seed, frame, pix_base, frame_dev, E_out,
+                           (float*)adc_out);
+    else
+        hipLaunchKernelGGL(photon_philox_kernel<double>, dim3((n + 255) / 256), dim3(256), 0,
+                           stream, n, photons, gain_map, offset_map, *params, table_values,
+                           table_cdf, table_n, seed, frame, pix_base, frame_dev, E_out,
+                           (double*)adc_out);
+    SMX_CHECK_LAUNCH("photon_philox_kernel");
     return SMX_OK;
 }
 

@@ -83,3 +83,66 @@ def test_form_image_errors(gpu):
         scopyon_amd.form_image(numpy.zeros(3), config=cfg, rng=numpy.random.RandomState(0))
     with pytest.raises(TypeError):
         scopyon_amd.form_image(3.0, config=cfg, rng=numpy.random.RandomState(0))
+
+
+def _cmos_table():
+    import os
+    import scopyon_amd
+    path = os.path.join(os.path.dirname(scopyon_amd.__file__), 'catalog', 'detector',
+                        'RNDist_F40.csv')
+    return numpy.loadtxt(path, delimiter=',')
+
+
+@pytest.mark.parametrize('kind', ['CCD', 'CMOS'])
+def test_ccd_and_cmos_match_oracle(gpu, kind):
+    """CCD / CMOS detectors (_epifm.py:329-353, 418-433, 1443-1469): same ADC
+    counts as the oracle for the same RandomState stream."""
+    import scopyon_amd
+    ov = {'detector': {'type': kind, 'image_size': [40, 36], 'exposure_time': 0.033},
+          'magnification': 100.0}
+    pl = 16e-6 / 100.0
+    rng = numpy.random.RandomState(9)
+    inputs = rng.uniform(-18 * pl, 18 * pl, size=(20, 2))
+    img, info = scopyon_amd.form_image(inputs, config=product_config(ov),
+                                       rng=numpy.random.RandomState(8), full_output=True)
+    orng = numpy.random.RandomState(8)
+    p = oracle_params(ov, orng)
+    data = list(O.format_inputs(inputs, PRE))
+    cam, _ = O.output_frame(p, O.PSFCache(p), data, rng=orng,
+                            cmos_table=_cmos_table() if kind == 'CMOS' else None)
+    numpy.testing.assert_allclose(info['expectation'], cam[:, :, 0], rtol=1e-10, atol=0)
+    assert numpy.array_equal(img.as_array(), cam[:, :, 1])
+
+
+@pytest.mark.parametrize('kind', ['CCD', 'CMOS'])
+def test_ccd_and_cmos_philox_moments(gpu, kind):
+    """Philox CCD / CMOS: photo-electrons Poisson(E) plus readout noise
+    (normal of the configured sigma, or the CMOS table's distribution):
+    mean and variance of the ADC counts with gain 1."""
+    import torch
+    from scopyon_amd import _lib, _device
+    n = 512 * 512
+    det = _lib.DetectorParamsT()
+    det.qe, det.background, det.background_switch, det.emgain = 1.0, 0.0, 0, 1
+    det.readout_noise = 3.0 if kind == 'CCD' else 0.0
+    det.fullwell, det.adc_max, det.offset, det.gain = 1e300, 1e300, 100.0, 1.0
+    if kind == 'CMOS':
+        t = _cmos_table()
+        p = t[:, 1] / t[:, 1].sum()
+        cdf = p.cumsum()
+        cdf /= cdf[-1]
+        tv, tc = (torch.as_tensor(a, dtype=torch.float64, device=gpu) for a in (t[:, 0], cdf))
+        nmean, nvar = float((t[:, 0] * p).sum()), float((t[:, 0] ** 2 * p).sum() - ((t[:, 0] * p).sum()) ** 2)
+        targs = (tv.data_ptr(), tc.data_ptr(), len(cdf))
+    else:
+        nmean, nvar = 0.0, 9.0
+        targs = (None, None, 0)
+    for E in (0.3, 5.0, 40.0):
+        photons = torch.full((n,), E, dtype=torch.float64, device=gpu)
+        adc = torch.empty(n, dtype=torch.float32, device=gpu)
+        _lib.call('smx_photon_detector_philox', n, photons.data_ptr(), None, None, det, *targs,
+                  77, 1, 0, None, None, adc.data_ptr(), 1, _device.stream())
+        x = adc.cpu().numpy().astype(float) - 100.0
+        var = E + nvar
+        assert abs(x.mean() - (E + nmean)) < 5 * numpy.sqrt(var / n), (kind, E, x.mean())
+        assert abs(x.var() / var - 1.0) < 0.03, (kind, E, x.var(), var)

@@ -73,3 +73,26 @@ def test_emccd_memo_does_not_change_draws():
     a = O.emccd_signal(E, 300, numpy.random.RandomState(4), memo=True)
     b = O.emccd_signal(E, 300, numpy.random.RandomState(4), memo=False)
     assert numpy.array_equal(a, b)
+
+
+def test_ccd_cmos_loops_equal_vectorised_draws():
+    """The reference's per-pixel CCD/CMOS loops (_epifm.py:331-353, 427-433)
+    draw exactly what one vectorised call draws: rng.poisson over the image in
+    C order, and for CMOS one random_sample per pixel mapped through
+    searchsorted(cdf, u, 'right') -- the forms the GPU path relies on."""
+    import os
+    import scopyon_amd
+    table = numpy.loadtxt(os.path.join(os.path.dirname(scopyon_amd.__file__), 'catalog',
+                                       'detector', 'RNDist_F40.csv'), delimiter=',')
+    E = numpy.random.RandomState(0).uniform(0, 30, (7, 9))
+    E[0, 0] = 0.0
+    a, b = numpy.random.RandomState(5), numpy.random.RandomState(5)
+    numpy.testing.assert_array_equal(O.poisson_signal(E, a), b.poisson(E).astype(float))
+    assert a.random_sample() == b.random_sample()
+    a, b = numpy.random.RandomState(6), numpy.random.RandomState(6)
+    got = O.cmos_noise((7, 9), a, table)
+    p = table[:, 1] / table[:, 1].sum()
+    cdf = p.cumsum()
+    cdf /= cdf[-1]
+    want = table[numpy.searchsorted(cdf, b.random_sample(63), side='right'), 0].reshape(7, 9)
+    numpy.testing.assert_array_equal(got, want)
