@@ -167,3 +167,9 @@ class PSFTables:
         """Build every depth key (0 .. cutoff) plus the cutoff key."""
         keys = list(range(self.n_keys - 2)) + [-1]
         self.ensure(keys)
+
+    def prebuild_range(self, max_abs_depth):
+        """Build the keys of every |depth| <= max_abs_depth (from the focal
+        plane) plus the cutoff key."""
+        top = min(self.n_keys - 2, int(max_abs_depth / RES) + 2)
+        self.ensure(list(range(top)) + [-1])

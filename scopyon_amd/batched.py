@@ -14,6 +14,8 @@ Layout in HBM (structure of arrays): emitter coordinates ``depth, u, v``
 ``pix_off`` offsets: photons/expectation (float64) and ADC counts (float32 by
 default).  The PSF summed-area tables are shared by all FOVs.
 """
+import os
+
 import numpy
 import torch
 
@@ -34,8 +36,9 @@ class BatchedSimulator:
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
         if not c.uniform_adc():
             raise NotImplementedError('fixed-pattern ADC maps are not supported in batches')
-        if c.detector_type != 'EMCCD':
-            raise NotImplementedError('batched simulation supports the EMCCD detector')
+        if c.detector_type not in ('EMCCD', 'CCD', 'CMOS'):
+            raise RuntimeError("Unknown detector type was given [{}]. ".format(c.detector_type)
+                               + "Use either one of 'CMOS', 'CCD' or 'EMCCD'.")
         self.dt = float(dt if dt is not None else c.detector_exposure_time)
         self.seed = int(seed)
         self.nfov = len(fov_shapes)
@@ -62,10 +65,22 @@ class BatchedSimulator:
         if depth_range is None:
             self.tables.ensure([0])
         else:
-            self.tables.prebuild_all()
+            # every 1 nm key an emitter inside the depth range (walls keep it
+            # there) can reach, plus the cutoff key
+            dmax = max(abs(depth_range[0]), abs(depth_range[1]))
+            self.tables.prebuild_range(dmax)
         self.slot_map = self.tables.slot_map()
         self.splatter = Splatter(c, self.tables, self.device)
         self.det = c.detector_params()
+        self._table = None
+        if c.detector_type == 'CMOS':
+            t = numpy.loadtxt(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'catalog',
+                                           'detector', 'RNDist_F40.csv'), delimiter=',')
+            p = t[:, 1] / t[:, 1].sum()
+            cdf = p.cumsum()
+            cdf /= cdf[-1]
+            self._table = (_device.upload(t[:, 0], device=self.device),
+                           _device.upload(cdf, device=self.device), len(cdf))
         self.boundary = {'none': 0, 'periodic': 1, 'reflect': 2}[boundary]
         # emitters: uniform in each FOV's box (default: the frame extent)
         hostrng = numpy.random.Generator(numpy.random.Philox(self.seed))
@@ -179,9 +194,17 @@ class BatchedSimulator:
         _lib.call('smx_counter_add', self.counter.data_ptr(), 1, _device.stream())
 
     def emccd(self):
-        _lib.call('smx_emccd_philox', self.n_pix, self.photons.data_ptr(), None, None, self.det,
-                  self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
-                  self.adc.data_ptr(), 1 if self.adc.dtype == torch.float32 else 0,
+        """Detector + ADC of every pixel (EMCCD, or CCD / CMOS by configuration)."""
+        f32 = 1 if self.adc.dtype == torch.float32 else 0
+        if self.configs.detector_type == 'EMCCD':
+            _lib.call('smx_emccd_philox', self.n_pix, self.photons.data_ptr(), None, None,
+                      self.det, self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
+                      self.adc.data_ptr(), f32, _device.stream())
+            return
+        tv, tc, tn = self._table if self._table is not None else (None, None, 0)
+        _lib.call('smx_photon_detector_philox', self.n_pix, self.photons.data_ptr(), None, None,
+                  self.det, _device.ptr(tv), _device.ptr(tc), tn, self.seed, 0, 0,
+                  self.counter.data_ptr(), _device.ptr(self.E), self.adc.data_ptr(), f32,
                   _device.stream())
 
     def set_positions(self, depth, u, v):
