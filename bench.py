@@ -129,7 +129,7 @@ def fwd_bwd(reps=20, warmup=3):
     I = dev(rng.uniform(500.0, 1500.0, n))
     fid = dev(numpy.repeat(numpy.arange(B), N), torch.int32)
     model = GaussianModel([(S, S)] * B, sigma=1.2, inv_depth=1.0 / 0.377, background=10.0,
-                          radius=6)
+                          radius=6, method='mfma-f32')
     mu_true = model.forward(dev(tx), dev(ty), z, I, fid)
     det = _lib.DetectorParamsT()
     det.qe, det.background, det.background_switch, det.emgain = 0.92, 0.0, 0, 300
@@ -184,12 +184,149 @@ def fwd_bwd(reps=20, warmup=3):
     model.check_status()
     alg = B * (52 * N + 12 * S * S)
     return {'workload': '64 frames 256x256, 5000 emitters each, Gaussian PSF sigma 1.2 px, '
-                        'Poisson NLL, backward for x, y, z, I, sigma; hipGraph replay',
+                        'forward on the matrix cores (float32 operands), Poisson NLL, '
+                        'backward for x, y, z, I, sigma; hipGraph replay',
             'fwd_bwd_per_sec': 1.0 / sec, 'frames_per_sec': B / sec, 'ms_per_iteration': sec * 1e3,
             'kernel_ms': times, 'loss': float(model.loss.sum()),
             'roofline': {'bound': 'hbm', 'achieved': alg / sec / 1e9, 'peak': HBM_PEAK_GBS,
                          'unit': 'GB/s', 'frac': alg / sec / 1e9 / HBM_PEAK_GBS,
                          'algorithmic_bytes': alg}}
+
+
+def precision_leg(reps=10):
+    """Separable Gaussian PSF (sigma 1.2 px) over 1,000,000 emitters uniform on
+    one 2048 x 2048 frame, intensities exponential (mean 500 photons): the
+    contraction Gx^T diag(I) Gy on the matrix cores with bfloat16, float16 and
+    float32 operands (float32 accumulation) and the float32 LDS splat.  Each
+    form's time per frame (binning included) and its error against the float64
+    oracle on a 256 x 256 crop: relative RMS ||mu - ref|| / ||ref|| and the
+    largest relative error over pixels above 1% of the crop maximum."""
+    import numpy
+    import torch
+    from oracle import gauss as OG
+    from scopyon_amd.gauss import GaussianModel
+    S, N, R, C = 2048, 1_000_000, 6, 256
+    rng = numpy.random.Generator(numpy.random.Philox(11))
+    x, y = rng.uniform(0, S, N), rng.uniform(0, S, N)
+    I = rng.exponential(500.0, N)
+    dev = lambda a: torch.as_tensor(a, dtype=torch.float32, device='cuda')
+    xd, yd, Id, zd = dev(x), dev(y), dev(I), dev(numpy.zeros(N))
+    x32, y32, I32 = (a.astype(numpy.float32).astype(float) for a in (x, y, I))
+    # float64 oracle on the crop [c0, c0 + C)^2 (integer shifts keep G exact)
+    c0 = S // 2 - C // 2
+    sel = ((x32 >= c0 - R - 1) & (x32 < c0 + C + R + 1) & (y32 >= c0 - R - 1) &
+           (y32 < c0 + C + R + 1))
+    kw = dict(sigma=1.2, inv_depth=0.0, background=0.0, radius=R)
+    ref = OG.render([(C, C)], x32[sel] - c0, y32[sel] - c0, numpy.zeros(sel.sum()), I32[sel],
+                    numpy.zeros(sel.sum(), int), **kw)[0]
+    big = ref > 0.01 * ref.max()
+    # executed MFMA work: every tile list padded to 16 emitters
+    lo_r = numpy.floor(x32).astype(int) - R
+    lo_c = numpy.floor(y32).astype(int) - R
+    r0, r1 = numpy.clip(lo_r, 0, S - 1) // 32, numpy.clip(lo_r + 2 * R, 0, S - 1) // 32
+    q0, q1 = numpy.clip(lo_c, 0, S - 1) // 32, numpy.clip(lo_c + 2 * R, 0, S - 1) // 32
+    counts = numpy.zeros((S // 32, S // 32), dtype=numpy.int64)
+    for dr in (0, 1):
+        for dc in (0, 1):
+            ok = (r0 + dr <= r1) & (q0 + dc <= q1)
+            numpy.add.at(counts, (r0[ok] + dr, q0[ok] + dc), 1)
+    padded = int(numpy.sum((counts + 15) // 16 * 16))
+    out = {'workload': '1,000,000 emitters on 2048x2048, Gaussian sigma 1.2 px, window 13 px, '
+                       'I ~ Exp(500)',
+           'dense_equivalent_flop': 2.0 * N * S * S, 'forms': {}}
+    for method in ('mfma-bf16', 'mfma-f16', 'mfma-f32', 'lds'):
+        m = GaussianModel([(S, S)], method=method, **kw)
+        zt = zd if method == 'lds' else None
+        mu = m.forward(xd, yd, zt, Id)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            m.forward(xd, yd, zt, Id, out=mu)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        m.check_status()
+        got = mu.view(S, S)[c0:c0 + C, c0:c0 + C].cpu().numpy().astype(float)
+        rms = float(numpy.linalg.norm(got - ref) / numpy.linalg.norm(ref))
+        mx = float((numpy.abs(got - ref)[big] / ref[big]).max())
+        entry = {'ms_per_frame': ms, 'frames_per_sec': 1e3 / ms, 'rel_rms_err': rms,
+                 'max_rel_err': mx}
+        if method != 'lds':
+            peak = {'mfma-bf16': 2.5e15, 'mfma-f16': 2.5e15, 'mfma-f32': 157.3e12}[method]
+            flop = 2.0 * 32 * 32 * padded
+            entry.update({'executed_mfma_flop': flop, 'peak_flops': peak,
+                          'mfma_utilisation': flop / (ms * 1e-3) / peak,
+                          'dense_equivalent_tflops': 2.0 * N * S * S / (ms * 1e-3) / 1e12})
+        out['forms'][method] = entry
+        del m
+    return out
+
+
+def _replay_rate(sim, steps, warmup):
+    import torch
+    sim.capture()
+    for _ in range(warmup):
+        sim.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.replay(steps)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    assert sim.status() & ~1 == 0
+    return sec
+
+
+def scale3d_leg(steps=10, warmup=2):
+    """EPIFM 3D scale-up: 100,000 emitters uniform in a 50 x 50 x 2 um box,
+    D lognormal (median 0.5 um^2/s, sigma_log 1.0), dt 10 ms, EPI
+    illumination, 1024 x 1024 EMCCD frames at 241x (66.4 nm pixels: the general
+    summed-area splat), a float64 Born-Wolf table for every 1 nm depth key of
+    the +-1 um range (1002 tables, 32 GB of HBM)."""
+    import numpy
+    import scopyon_amd
+    from scopyon_amd.batched import BatchedSimulator
+    cfg = scopyon_amd.DefaultConfiguration()
+    cfg.update({'default': {'detector': {'image_size': [1024, 1024], 'exposure_time': 0.01},
+                            'light_source': {'angle': 0.0},
+                            'effects': {'photo_bleaching': {'switch': False}}}})
+    n = 100_000
+    rng = numpy.random.Generator(numpy.random.Philox(13))
+    D = numpy.exp(rng.normal(numpy.log(0.5e-12), 1.0, n))
+    t0 = time.perf_counter()
+    sim = BatchedSimulator(cfg, [(1024, 1024)], n, D=D, dt=0.01, seed=21,
+                           box=(50e-6, 50e-6), depth_range=(-1e-6, 1e-6))
+    import torch
+    torch.cuda.synchronize()
+    build = time.perf_counter() - t0
+    sec = _replay_rate(sim, steps, warmup)
+    return {'workload': '1 FOV 1024x1024 at 241x, 100,000 emitters in 50x50x2 um, lognormal D, '
+                        'EPI, dt 10 ms, 1002 depth tables',
+            'frames_per_sec': 1.0 / sec, 'ms_per_frame': sec * 1e3,
+            'table_build_s': build, 'depth_tables': len(sim.tables.slot_of),
+            'kernel_ms': _kernel_times(sim, 5)}
+
+
+def batched_uneven_leg(steps=20, warmup=3):
+    """Batched fields of view: 256 independent simulations, frame sizes drawn
+    from {128^2, 256^2, 512^2}, emitter counts lognormal (median 5000,
+    sigma_log 1.0) clipped to [100, 50000], Philox per FOV, one HIP graph per
+    step."""
+    import numpy
+    from scopyon_amd.batched import BatchedSimulator
+    rng = numpy.random.Generator(numpy.random.Philox(17))
+    F = 256
+    sizes = rng.choice([128, 256, 512], F)
+    counts = numpy.clip(numpy.exp(rng.normal(numpy.log(5000.0), 1.0, F)), 100, 50000).astype(int)
+    sim = BatchedSimulator(_config(512, 0.033), [(int(s), int(s)) for s in sizes], counts,
+                           D=0.1e-12, dt=0.033, seed=23)
+    sec = _replay_rate(sim, steps, warmup)
+    return {'workload': '256 FOVs, sizes from {128,256,512}^2, lognormal emitter counts '
+                        '(median 5000) clipped to [100, 50000], hipGraph replay',
+            'fovs': F, 'emitters': int(counts.sum()), 'pixels': int(sim.n_pix),
+            'frames_per_sec': F / sec, 'ms_per_step': sec * 1e3,
+            'emitter_frames_per_sec': float(counts.sum()) / sec,
+            'status_unsorted_lists': bool(sim.status() & 1)}
 
 
 def cpu_baseline(size, n_emit, sample_emitters, seed=1):
@@ -347,6 +484,9 @@ def main():
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': _traffic(dom),
                      'algorithmic_bytes': sbytes[dom], 'launch_ms': kt[dom]},
     }
+    result['batched_uneven'] = batched_uneven_leg()
+    result['scale_3d'] = scale3d_leg()
+    result['precision'] = precision_leg()
     result['fwd_bwd'] = fwd_bwd()
     result['fwd_bwd_per_sec'] = result['fwd_bwd']['fwd_bwd_per_sec']
     if not args.no_cpu:
