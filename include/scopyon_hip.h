@@ -231,7 +231,7 @@ int smx_gauss_poisson_nll(const float* mu, const float* k, const smx_fov_t* fram
 /* Hand-written backward of smx_gauss_forward: given grad_mu = dL/dmu, writes
  * dL/dx, dL/dy, dL/dz, dL/dI per emitter (any output may be NULL) and, when
  * gsigma (device scalar) is not NULL, dL/dsigma summed over all emitters in a
- * fixed order (float64); workspace then holds at least 8 * n bytes. */
+ * fixed order (float64); workspace then holds at least 8 * (n + 256) bytes. */
 int smx_gauss_backward(int n, const float* x, const float* y, const float* z, const float* I,
                        const int* frame_id, const smx_fov_t* frames_dev, int nframes,
                        const smx_gauss_params_t* params /* host */, const float* grad_mu,

@@ -383,19 +383,22 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
     }
 }
 
-// Deterministic float64 sum of n values by one workgroup (fixed assignment).
-__global__ void __launch_bounds__(1024) sum_f64_kernel(const double* __restrict__ in, int n,
-                                                       double* __restrict__ out) {
-    __shared__ double s_red[1024];
+// Deterministic float64 sum (fixed assignment of values to threads and a
+// fixed tree): block b sums in[b * per .. (b + 1) * per) into out[b].
+constexpr int SUM_BLOCKS = 256;
+__global__ void __launch_bounds__(256) sum_f64_kernel(const double* __restrict__ in, int n,
+                                                      int per, double* __restrict__ out) {
+    __shared__ double s_red[256];
+    const int start = blockIdx.x * per, stop = min(n, start + per);
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += in[i];
+    for (int i = start + threadIdx.x; i < stop; i += blockDim.x) s += in[i];
     s_red[threadIdx.x] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) s_red[threadIdx.x] += s_red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = s_red[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = s_red[0];
 }
 
 // --------------------------------------------------------------------------
@@ -765,13 +768,14 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
                        size_t workspace_bytes, hipStream_t stream) {
     if (n < 0 || nframes <= 0 || !frames_dev || !grad_mu || !params_ok(params) ||
         (n > 0 && (!x || !y || !z || !I)) ||
-        (gsigma && (!workspace || workspace_bytes < sizeof(double) * (size_t)(n > 0 ? n : 1)))) {
+        (gsigma && (!workspace ||
+                    workspace_bytes < sizeof(double) * ((size_t)(n > 0 ? n : 1) + SUM_BLOCKS)))) {
         smx::set_error("smx_gauss_backward: invalid arguments (n=%d, nframes=%d)", n, nframes);
         return SMX_EVALUE;
     }
     if (n == 0) {
-        if (gsigma) hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, stream,
-                                       (const double*)workspace, 0, gsigma);
+        if (gsigma) hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, stream,
+                                       (const double*)workspace, 0, 1, gsigma);
         return SMX_OK;
     }
     BwdArgs B;
@@ -781,8 +785,15 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
     hipLaunchKernelGGL(gauss_backward_kernel, dim3((n + 15) / 16), dim3(256), 0, stream, B);
     SMX_CHECK_LAUNCH("gauss_backward_kernel");
     if (gsigma) {
-        hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, stream,
-                           (const double*)workspace, n, gsigma);
+        // two passes: SUM_BLOCKS partial sums after the n per-emitter terms,
+        // then their sum
+        double* part = (double*)workspace + n;
+        const int per = (n + SUM_BLOCKS - 1) / SUM_BLOCKS;
+        hipLaunchKernelGGL(sum_f64_kernel, dim3(SUM_BLOCKS), dim3(256), 0, stream,
+                           (const double*)workspace, n, per, part);
+        SMX_CHECK_LAUNCH("sum_f64_kernel");
+        hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, stream, (const double*)part,
+                           SUM_BLOCKS, SUM_BLOCKS, gsigma);
         SMX_CHECK_LAUNCH("sum_f64_kernel");
     }
     return SMX_OK;
