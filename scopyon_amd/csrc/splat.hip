@@ -581,6 +581,32 @@ constexpr int QS = 16;            // quadrant side
 constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
 constexpr int PSORT_CAP = 2048;   // tile list sorted in LDS
 
+// Ascending bitonic sort of 128 ints held by one wave, element `lane` in a0
+// and element `lane + 64` in a1 (pad with INT_MAX); written to out[0..127].
+// Stage (k, j) pairs element p with p ^ j; the lower of the pair keeps the
+// minimum when p & k == 0 (ascending block), the maximum otherwise.
+__device__ inline void wave_sort128(int a0, int a1, int lane, int* out) {
+#pragma unroll
+    for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j == 64) {           // partner of element lane is lane + 64 (k == 128)
+                const int lo = min(a0, a1), hi = max(a0, a1);
+                a0 = lo;
+                a1 = hi;
+            } else {
+                const int b0 = __shfl_xor(a0, j), b1 = __shfl_xor(a1, j);
+                const bool lower = (lane & j) == 0;
+                const bool up0 = (lane & k) == 0, up1 = ((lane + 64) & k) == 0;
+                a0 = (lower == up0) ? min(a0, b0) : max(a0, b0);
+                a1 = (lower == up1) ? min(a1, b1) : max(a1, b1);
+            }
+        }
+    }
+    out[lane] = a0;
+    out[lane + 64] = a1;
+}
+
 template <typename KT>
 __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     __shared__ int s_sort[PSORT_CAP];
@@ -612,12 +638,19 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && sorted) {
+    if (count >= 1 && count <= 128) {
+        // short list: one wave sorts it in registers (two entries per lane)
+        if (wave == 0) {
+            wave_sort128(lane < count ? list[lane] : 0x7fffffff,
+                         lane + 64 < count ? list[lane + 64] : 0x7fffffff, lane, s_sort);
+        }
+        __syncthreads();
+    } else if (count >= 1 && sorted) {
         int np2 = 1;
         while (np2 < count) np2 <<= 1;
         for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
         __syncthreads();
-        if (count > 1) lds_bitonic_sort(s_sort, np2);
+        lds_bitonic_sort(s_sort, np2);
     } else if (!sorted && threadIdx.x == 0) {
         atomicOr(A.status, 1);
     }
@@ -692,24 +725,26 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
                 }
             } else {
                 for (int b = 0; b < total; b += NSLOT) {
+                    // branch-free part decoding (24-bit multiplies, masked
+                    // offsets), so the NSLOT table loads issue back to back
                     KT kv[NSLOT];
                     double wv[NSLOT];
                     int pix[NSLOT];
 #pragma unroll
                     for (int s = 0; s < NSLOT; ++s) {
-                        pix[s] = -1;
-                        if (b + s < total) {
-                            const int ent = s_part[wave][b + s], c = ent >> 2;
-                            const int4 I = s_qi[wave][c];
-                            const int idx = (ent & 3) * 64 + lane;
-                            const bool ok = idx < I.z;
-                            const int W = I.y & 31, magic = I.y >> 5;
-                            const int row = (idx * magic) >> 16, col = idx - row * W;
-                            const int su = (short)(I.w & 0xffff), sv = I.w >> 16;
-                            kv[s] = smx::gload(s_qk[wave][c] + (ok ? row * su + col * sv : 0));
-                            wv[s] = s_w[c];
-                            pix[s] = ok ? I.x + row * QS + col : -1;
-                        }
+                        const int ent = s_part[wave][min(b + s, total - 1)], c = ent >> 2;
+                        const int4 I = s_qi[wave][c];
+                        const KT* kq = s_qk[wave][c];
+                        wv[s] = s_w[c];
+                        const int idx = (ent & 3) * 64 + lane;
+                        const bool ok = (b + s < total) & (idx < I.z);
+                        const int W = I.y & 31, magic = I.y >> 5;
+                        const int row = (int)(__umul24((unsigned)idx, (unsigned)magic) >> 16);
+                        const int col = idx - __mul24(row, W);
+                        const int su = (short)(I.w & 0xffff), sv = I.w >> 16;
+                        const int off = (__mul24(row, su) + __mul24(col, sv)) & -(int)ok;
+                        kv[s] = smx::gload(kq + off);
+                        pix[s] = ok ? I.x + row * QS + col : -1;
                     }
 #pragma unroll
                     for (int s = 0; s < NSLOT; ++s)
