@@ -168,6 +168,13 @@ __device__ inline float normal_f(uint32_t a, uint32_t b) {
                                            __builtin_amdgcn_logf(u01f_open0(a)));
     return r * __builtin_amdgcn_cosf(u01f(b));
 }
+// Both Box-Muller normals of one word pair (independent N(0, 1) variates).
+__device__ inline float2 normal2_f(uint32_t a, uint32_t b) {
+    const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f *
+                                           __builtin_amdgcn_logf(u01f_open0(a)));
+    const float t = u01f(b);
+    return make_float2(r * __builtin_amdgcn_cosf(t), r * __builtin_amdgcn_sinf(t));
+}
 
 // Poisson for lam >= 12: Hormann's PTRS (transformed rejection with squeeze,
 // Insurance: Mathematics and Economics 12, 1993).
@@ -194,16 +201,27 @@ __device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
 
 // Gamma(n, 1), n >= 1: Marsaglia-Tsang (ACM TOMS 26(3), 2000), one Philox
 // block per iteration (normal from two words, uniform from the third).
-__device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk) {
+// With `first`, the first iteration takes its normal x0 and uniform u0 from
+// the caller (the spare Box-Muller normal and spare word of the pixel's block
+// 0), so an accepted first proposal -- the common case -- costs no further
+// Philox block.
+__device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk, bool first, float x0,
+                                float u0) {
     const float d = (float)n - 1.0f / 3.0f;
     const float c = 1.0f / sqrtf(9.0f * d);
     for (int it = 0; it < 256; ++it) {
-        smx::u32x4 r = R.block(blk++);
-        float x = normal_f(r.x, r.y);
+        float x, u;
+        if (it == 0 && first) {
+            x = x0;
+            u = u0;
+        } else {
+            smx::u32x4 r = R.block(blk++);
+            x = normal_f(r.x, r.y);
+            u = u01f_open0(r.z);
+        }
         float v = 1.0f + c * x;
         if (v <= 0.0f) continue;
         v = v * v * v;
-        float u = u01f_open0(r.z);
         if (u < 1.0f - 0.0331f * (x * x) * (x * x)) return d * v;
         if (ln_hw(u) < 0.5f * x * x + d * (1.0f - v + ln_hw(v))) return d * v;
     }
@@ -226,8 +244,11 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
     const float E = (float)P.qe * ph;
     PixelRng R{pix_base + (uint32_t)i, frame, (uint32_t)seed, (uint32_t)(seed >> 32)};
     const smx::u32x4 r0 = R.block(0);
+    // block 0: (x, y) -> readout-noise normal and a spare normal for the first
+    // Gamma proposal, z -> Poisson uniform, w -> first Gamma uniform
+    const float2 nn = normal2_f(r0.x, r0.y);
     float noise = 0.0f;
-    if (P.readout_noise > 0.0) noise = (float)P.readout_noise * normal_f(r0.x, r0.y);
+    if (P.readout_noise > 0.0) noise = (float)P.readout_noise * nn.x;
     float S = 0.0f;
     if (E > 0.0f) {
         const float lam = E;
@@ -251,7 +272,7 @@ __global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
             uint32_t blk = 1u << 16;
             float s = 0.0f;
             for (int guard = 0; guard < 64; ++guard) {
-                s = gamma_mt(R, k, blk) * (float)P.emgain;
+                s = gamma_mt(R, k, blk, guard == 0, nn.y, u01f_open0(r0.w)) * (float)P.emgain;
                 if (s >= 0.5f) break;
             }
             S = floorf(s + 0.5f);
