@@ -381,6 +381,16 @@ struct EmitterRec {
     long long table_off;     // element of pixel (du, dv) = table_off + du * su + dv * sv
 };
 
+// Tiles touched by a record's footprint rectangle [lo, lo + n) clipped to the frame.
+__device__ inline bool rec_tiles(const smx_fov_t& F, const EmitterRec& r, TileRange& t) {
+    const int r0 = max(r.lo_u, 0), c0 = max(r.lo_v, 0);
+    const int r1 = (int)min((long long)r.lo_u + r.n_u, (long long)F.nw) - 1;
+    const int c1 = (int)min((long long)r.lo_v + r.n_v, (long long)F.nh) - 1;
+    if (r1 < r0 || c1 < c0) return false;
+    t.r0 = r0 / TS; t.r1 = r1 / TS; t.c0 = c0 / TS; t.c1 = c1 / TS;
+    return true;
+}
+
 __device__ inline int regular_phase(const Axis& a, double half, double xi, double ext_half,
                                     double pl, double res, int ntab, int P, int M,
                                     long long lo, long long hi, bool& regular) {
@@ -404,7 +414,8 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
                                    const int* __restrict__ fov_id, const smx_fov_t* __restrict__ fovs,
                                    int nfov, double ext_half, double res, int ntab, int P, int M,
                                    int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
-                                   EmitterRec* __restrict__ rec) {
+                                   EmitterRec* __restrict__ rec, int n_tiles,
+                                   int* __restrict__ tile_count, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     EmitterRec r;
@@ -420,6 +431,7 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
     if (!fov_ok(f, nfov) || slot < 0 || slot >= n_slots) {
         r.flags = 2;
         rec[i] = r;
+        if (fov_ok(f, nfov) && r.w > 0.0) atomicOr(status, 8);
         return;
     }
     const smx_fov_t F = fovs[f];
@@ -471,6 +483,43 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
         r.table_off = (long long)slot * (long long)k_stride + off;
     }
     rec[i] = r;
+    if (tile_count && r.w > 0.0) {
+        TileRange t;
+        if (rec_tiles(F, r, t))
+            for (int a = t.r0; a <= t.r1; ++a)
+                for (int b = t.c0; b <= t.c1; ++b) {
+                    const int tile = F.tile_off + a * F.tiles_c + b;
+                    if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[tile], 1);
+                }
+    }
+}
+
+// Tile lists from the records (phase path): the footprint is the record's
+// pixel rectangle, already computed with the reference's float64 boundaries.
+__global__ void bin_fill_rec_kernel(int n, const EmitterRec* __restrict__ rec,
+                                    const int* __restrict__ fov_id,
+                                    const smx_fov_t* __restrict__ fovs, int nfov, int n_tiles,
+                                    const int* __restrict__ tile_begin,
+                                    int* __restrict__ tile_cursor, int* __restrict__ list,
+                                    long long capacity, int* __restrict__ status) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int f = fov_of(fov_id, i);
+    if (!fov_ok(f, nfov)) return;
+    const EmitterRec r = rec[i];
+    if (r.flags == 2 || !(r.w > 0.0)) return;
+    const smx_fov_t F = fovs[f];
+    TileRange t;
+    if (!rec_tiles(F, r, t)) return;
+    for (int a = t.r0; a <= t.r1; ++a)
+        for (int b = t.c0; b <= t.c1; ++b) {
+            const int tile = F.tile_off + a * F.tiles_c + b;
+            if (tile < 0 || tile >= n_tiles) continue;
+            const int pos = atomicAdd(&tile_cursor[tile], 1);
+            const long long at = (long long)tile_begin[tile] + pos;
+            if (at < capacity) list[at] = i;
+            else atomicOr(status, 2);
+        }
 }
 
 // K[slot][phi_u][phi_v][du][dv] from the SATs (same arithmetic as the tile kernel).
@@ -907,9 +956,20 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                            ws.tile_fov, n_tiles);
         SMX_CHECK_LAUNCH("tile_fov_kernel");
         if (n > 0) {
-            hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, fov_id,
-                               fovs_dev, nfov, ext_half, n_tiles, ws.tile_count);
-            SMX_CHECK_LAUNCH("bin_count_kernel");
+            if (K) {
+                // records first; they count their tiles as they are written
+                const int folded = k_format == 2;
+                hipLaunchKernelGGL(emitter_rec_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
+                                   key, fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m,
+                                   n_slots,
+                                   folded ? PF.elems() : smx_phase_table_elems(sat_period, m),
+                                   folded, PF, ws.rec, n_tiles, ws.tile_count, status_dev);
+                SMX_CHECK_LAUNCH("emitter_rec_kernel");
+            } else {
+                hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
+                                   fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_count);
+                SMX_CHECK_LAUNCH("bin_count_kernel");
+            }
         }
         size_t cb = ws.cub_bytes;
         if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_count,
@@ -917,18 +977,15 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                            "hipcub ExclusiveSum"))
             return SMX_ERUNTIME;
         if (n > 0) {
-            hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, fov_id,
-                               fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin, ws.tile_cursor, ws.list,
-                               list_capacity, status_dev);
+            if (K)
+                hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nb), dim3(256), 0, stream, n, ws.rec,
+                                   fov_id, fovs_dev, nfov, n_tiles, ws.tile_begin,
+                                   ws.tile_cursor, ws.list, list_capacity, status_dev);
+            else
+                hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
+                                   fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin,
+                                   ws.tile_cursor, ws.list, list_capacity, status_dev);
             SMX_CHECK_LAUNCH("bin_fill_kernel");
-        }
-        if (K && n > 0) {
-            const int folded = k_format == 2;
-            hipLaunchKernelGGL(emitter_rec_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w, key,
-                               fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m, n_slots,
-                               folded ? PF.elems() : smx_phase_table_elems(sat_period, m), folded,
-                               PF, ws.rec);
-            SMX_CHECK_LAUNCH("emitter_rec_kernel");
         }
     }
     if (!(stages & 2)) return SMX_OK;
