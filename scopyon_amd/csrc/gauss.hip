@@ -99,7 +99,7 @@ __global__ void gauss_bin_fill_kernel(int n, const float* __restrict__ x, const 
                                       const int* __restrict__ frame_id,
                                       const smx_fov_t* __restrict__ frames, int nframes, int R,
                                       int n_tiles, const int* __restrict__ tile_begin,
-                                      int* __restrict__ tile_cursor, int* __restrict__ list,
+                                      int* __restrict__ tile_left, int* __restrict__ list,
                                       long long capacity, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -112,7 +112,7 @@ __global__ void gauss_bin_fill_kernel(int n, const float* __restrict__ x, const 
         for (int c = t.c0; c <= t.c1; ++c) {
             int tile = F.tile_off + r * F.tiles_c + c;
             if (tile < 0 || tile >= n_tiles) continue;
-            int pos = atomicAdd(&tile_cursor[tile], 1);
+            int pos = atomicSub(&tile_left[tile], 1) - 1;   // counts run down to 0
             long long at = (long long)tile_begin[tile] + pos;
             if (at < capacity) list[at] = i;
             else atomicOr(status, 2);
@@ -563,7 +563,6 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
 struct GaussWorkspace {
     int* tile_count;   // n_tiles + 1
     int* tile_begin;   // n_tiles + 1
-    int* tile_cursor;  // n_tiles
     int* tile_fov;     // n_tiles
     int* list;         // capacity
     double* gsig;      // n
@@ -578,7 +577,6 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
     size_t o_count = take(sizeof(int) * (n_tiles + 1));
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
-    size_t o_cursor = take(sizeof(int) * n_tiles);
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
     size_t o_sig = take(sizeof(double) * (size_t)(n > 0 ? n : 1));
@@ -590,7 +588,6 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     if (ws && base) {
         ws->tile_count = (int*)(base + o_count);
         ws->tile_begin = (int*)(base + o_begin);
-        ws->tile_cursor = (int*)(base + o_cursor);
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
         ws->gsig = (double*)(base + o_sig);
@@ -627,18 +624,9 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
               const smx_fov_t* frames_dev, int nframes, int n_tiles, int R, long long capacity,
               GaussWorkspace& ws, int* status_dev, hipStream_t stream) {
     const int nb = (n + 255) / 256;
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0, stream,
-                       ws.tile_count, (size_t)n_tiles + 1, 0);
-    SMX_CHECK_LAUNCH("fill_int_kernel");
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
-                       ws.tile_cursor, (size_t)n_tiles, 0);
-    SMX_CHECK_LAUNCH("fill_int_kernel");
-    hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
-                       ws.tile_fov, (size_t)n_tiles, -1);
-    SMX_CHECK_LAUNCH("fill_int_kernel");
-    hipLaunchKernelGGL(tile_fov_kernel, dim3(nframes), dim3(256), 0, stream, nframes, frames_dev,
-                       ws.tile_fov, n_tiles);
-    SMX_CHECK_LAUNCH("tile_fov_kernel");
+    hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
+                       stream, nframes, frames_dev, n_tiles, ws.tile_count, ws.tile_fov);
+    SMX_CHECK_LAUNCH("tiles_init_kernel");
     if (n > 0) {
         hipLaunchKernelGGL(gauss_bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, x, y,
                            frame_id, frames_dev, nframes, R, n_tiles, ws.tile_count);
@@ -652,7 +640,7 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
     if (n > 0) {
         hipLaunchKernelGGL(gauss_bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, x, y,
                            frame_id, frames_dev, nframes, R, n_tiles, ws.tile_begin,
-                           ws.tile_cursor, ws.list, capacity, status_dev);
+                           ws.tile_count, ws.list, capacity, status_dev);
         SMX_CHECK_LAUNCH("gauss_bin_fill_kernel");
     }
     return SMX_OK;

@@ -145,7 +145,7 @@ __global__ void bin_fill_kernel(int n, const double* __restrict__ u, const doubl
                                 const double* __restrict__ w, const int* __restrict__ fov_id,
                                 const smx_fov_t* __restrict__ fovs, int nfov, double ext_half,
                                 int n_tiles, const int* __restrict__ tile_begin,
-                                int* __restrict__ tile_cursor,
+                                int* __restrict__ tile_left,
                                 int* __restrict__ list, long long capacity, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -159,7 +159,7 @@ __global__ void bin_fill_kernel(int n, const double* __restrict__ u, const doubl
         for (int c = t.c0; c <= t.c1; ++c) {
             int tile = F.tile_off + r * F.tiles_c + c;
             if (tile < 0 || tile >= n_tiles) continue;
-            int pos = atomicAdd(&tile_cursor[tile], 1);
+            int pos = atomicSub(&tile_left[tile], 1) - 1;   // counts run down to 0
             long long at = (long long)tile_begin[tile] + pos;
             if (at < capacity) list[at] = i;
             else atomicOr(status, 2);
@@ -500,7 +500,7 @@ __global__ void bin_fill_rec_kernel(int n, const EmitterRec* __restrict__ rec,
                                     const int* __restrict__ fov_id,
                                     const smx_fov_t* __restrict__ fovs, int nfov, int n_tiles,
                                     const int* __restrict__ tile_begin,
-                                    int* __restrict__ tile_cursor, int* __restrict__ list,
+                                    int* __restrict__ tile_left, int* __restrict__ list,
                                     long long capacity, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -515,7 +515,7 @@ __global__ void bin_fill_rec_kernel(int n, const EmitterRec* __restrict__ rec,
         for (int b = t.c0; b <= t.c1; ++b) {
             const int tile = F.tile_off + a * F.tiles_c + b;
             if (tile < 0 || tile >= n_tiles) continue;
-            const int pos = atomicAdd(&tile_cursor[tile], 1);
+            const int pos = atomicSub(&tile_left[tile], 1) - 1;   // counts run down to 0
             const long long at = (long long)tile_begin[tile] + pos;
             if (at < capacity) list[at] = i;
             else atomicOr(status, 2);
@@ -822,7 +822,6 @@ struct SplatWorkspace {
     EmitterRec* rec;   // n (phase-table path)
     int* tile_count;   // n_tiles + 1
     int* tile_begin;   // n_tiles + 1
-    int* tile_cursor;  // n_tiles
     int* tile_fov;     // n_tiles
     int* list;         // capacity
     void* cub_tmp;
@@ -841,7 +840,6 @@ size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, Spla
     size_t o_rec = take(sizeof(EmitterRec) * (size_t)(n > 0 ? n : 1));
     size_t o_count = take(sizeof(int) * (n_tiles + 1));
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
-    size_t o_cursor = take(sizeof(int) * n_tiles);
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
     size_t cb = scan_tmp_bytes(n_tiles);
@@ -850,7 +848,6 @@ size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, Spla
         ws->rec = (EmitterRec*)(base + o_rec);
         ws->tile_count = (int*)(base + o_count);
         ws->tile_begin = (int*)(base + o_begin);
-        ws->tile_cursor = (int*)(base + o_cursor);
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
         ws->cub_tmp = base + o_cub;
@@ -943,18 +940,9 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     const int nb = (n + 255) / 256;
 
     if (stages & 1) {
-        hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0, stream,
-                           ws.tile_count, (size_t)n_tiles + 1, 0);
-        SMX_CHECK_LAUNCH("fill_int_kernel");
-        hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
-                           ws.tile_cursor, (size_t)n_tiles, 0);
-        SMX_CHECK_LAUNCH("fill_int_kernel");
-        hipLaunchKernelGGL(fill_int_kernel, dim3((n_tiles + 255) / 256), dim3(256), 0, stream,
-                           ws.tile_fov, (size_t)n_tiles, -1);
-        SMX_CHECK_LAUNCH("fill_int_kernel");
-        hipLaunchKernelGGL(tile_fov_kernel, dim3(nfov), dim3(256), 0, stream, nfov, fovs_dev,
-                           ws.tile_fov, n_tiles);
-        SMX_CHECK_LAUNCH("tile_fov_kernel");
+        hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
+                           stream, nfov, fovs_dev, n_tiles, ws.tile_count, ws.tile_fov);
+        SMX_CHECK_LAUNCH("tiles_init_kernel");
         if (n > 0) {
             if (K) {
                 // records first; they count their tiles as they are written
@@ -980,11 +968,11 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
             if (K)
                 hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nb), dim3(256), 0, stream, n, ws.rec,
                                    fov_id, fovs_dev, nfov, n_tiles, ws.tile_begin,
-                                   ws.tile_cursor, ws.list, list_capacity, status_dev);
+                                   ws.tile_count, ws.list, list_capacity, status_dev);
             else
                 hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
                                    fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin,
-                                   ws.tile_cursor, ws.list, list_capacity, status_dev);
+                                   ws.tile_count, ws.list, list_capacity, status_dev);
             SMX_CHECK_LAUNCH("bin_fill_kernel");
         }
     }

@@ -28,6 +28,54 @@ __global__ void tile_fov_kernel(int nfov, const smx_fov_t* __restrict__ fovs,
     }
 }
 
+// One pass before binning: tile_count[0..n_tiles] = 0 and tile_fov[g] = the
+// frame owning global tile g (-1 where none does).  The frames' tile ranges
+// are staged in LDS (up to TILES_INIT_FOVS of them) and searched by bisection
+// when their offsets ascend, linearly otherwise.
+constexpr int TILES_INIT_FOVS = 2048;
+__global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
+                                                         const smx_fov_t* __restrict__ fovs,
+                                                         int n_tiles, int* __restrict__ tile_count,
+                                                         int* __restrict__ tile_fov) {
+    __shared__ int s_off[TILES_INIT_FOVS], s_end[TILES_INIT_FOVS];
+    __shared__ int s_unsorted;
+    const int staged = nfov < TILES_INIT_FOVS ? nfov : TILES_INIT_FOVS;
+    if (threadIdx.x == 0) s_unsorted = 0;
+    __syncthreads();
+    for (int f = threadIdx.x; f < staged; f += blockDim.x) {
+        s_off[f] = fovs[f].tile_off;
+        s_end[f] = fovs[f].tile_off + fovs[f].tiles_r * fovs[f].tiles_c;
+    }
+    __syncthreads();
+    for (int f = threadIdx.x + 1; f < staged; f += blockDim.x)
+        if (s_off[f] < s_end[f - 1]) s_unsorted = 1;
+    __syncthreads();
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > n_tiles) return;
+    tile_count[g] = 0;
+    if (g == n_tiles) return;
+    int owner = -1;
+    if (!s_unsorted && staged == nfov) {
+        int lo = 0, hi = nfov;   // last frame with s_off <= g
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        if (nfov > 0 && g >= s_off[lo] && g < s_end[lo]) owner = lo;
+    } else {
+        for (int f = 0; f < nfov; ++f) {
+            const int off = f < staged ? s_off[f] : fovs[f].tile_off;
+            const int end = f < staged ? s_end[f] : off + fovs[f].tiles_r * fovs[f].tiles_c;
+            if (g >= off && g < end) {
+                owner = f;
+                break;
+            }
+        }
+    }
+    tile_fov[g] = owner;
+}
+
 // Ascending bitonic sort of n_pow2 ints in LDS by the whole workgroup.
 __device__ inline void lds_bitonic_sort(int* a, int n_pow2) {
     for (int k = 2; k <= n_pow2; k <<= 1) {
