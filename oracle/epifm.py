@@ -309,9 +309,15 @@ def overlay_molecule(p, psf, expected, particle, p_0, unit_time, optinfo, states
             budget = 0
             p_state = 0.0
         states[m_id] = budget
-    if p.dichroic_switch or p.emission_switch:
-        raise NotImplementedError('dichroic/emission filter spectra are not restated')
-    normalization = numpy.sum(p.fluoem_norm) * p.psf_normalization
+    # _epifm.py:1308-1315: fluoem_norm is scaled in place; an efficiency read
+    # from a catalog is a Python list (_epifm.py:996), so 0.01 * eff raises
+    # TypeError; without a type it is numpy zeros (_epifm.py:877, 980)
+    normalization = p.fluoem_norm
+    if p.dichroic_switch:
+        normalization *= 0.01 * p.dichroic_eff
+    if p.emission_switch:
+        normalization *= 0.01 * p.emission_eff
+    normalization = numpy.sum(normalization) * p.psf_normalization
     normalization *= p_state * N_emit / (4.0 * numpy.pi)
     pixel_length = p.pixel_length / p.magnification
     overlay_signal(psf, expected, p_i - numpy.asarray(p_0), pixel_length, normalization)

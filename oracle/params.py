@@ -146,6 +146,13 @@ def epifm_params(cfg, rng=None):
     p.light_source = cfg['light_source']
     p.dichroic_switch = cfg['dichroic_mirror']['switch']
     p.emission_switch = cfg['emission_filter']['switch']
+    # _epifm.py:871-877, 974-980: a catalog type gives a list (tolist()), none
+    # gives zeros; the list's values never matter (0.01 * list raises first)
+    nwl = len(p.fluoem_norm)
+    p.dichroic_eff = ([0.0] * nwl if cfg['dichroic_mirror'].get('type') is not None
+                      else numpy.zeros(nwl))
+    p.emission_eff = ([0.0] * nwl if cfg['emission_filter'].get('type') is not None
+                      else numpy.zeros(nwl))
     p.magnification = cfg['magnification']
     det = cfg['detector']
     p.detector_type = det['type']

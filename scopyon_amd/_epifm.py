@@ -96,9 +96,16 @@ class EPIFMConfigs:
         self.source_angle = ls.angle
         self.dichroic_switch = config.dichroic_mirror.switch
         self.emission_switch = config.emission_filter.switch
-        if self.dichroic_switch or self.emission_switch:
-            raise NotImplementedError(
-                'dichroic_mirror / emission_filter spectra are not supported by scopyon_amd')
+        # _epifm.py:1308-1313 scales fluoem_norm in place by 0.01 * efficiency.
+        # An efficiency read from a catalog is a Python list (_epifm.py:996),
+        # so the reference raises TypeError at the first emitter; without a
+        # type the efficiency is zeros and every weight becomes 0 for good.
+        self.filter_error = None
+        typed = ((self.dichroic_switch and config.dichroic_mirror.type is not None) or
+                 (self.emission_switch and config.emission_filter.type is not None))
+        if typed:
+            self.filter_error = "can't multiply sequence by non-int of type 'float'"
+        self.spectral_zero = self.dichroic_switch or self.emission_switch
         self.image_magnification = config.magnification
         det = config.detector
         self.detector_type = det.type
@@ -191,7 +198,8 @@ class EPIFMConfigs:
         p.x_sec = float(x_sec)
         p.quantum_yield = float(self.quantum_yield)
         p.one_minus_10_pow_minus_a = float(one_minus)
-        p.spectral_psf_norm = float(numpy.sum(self.fluoem_norm) * self.psf_normalization)
+        spectral = 0.0 * self.fluoem_norm if self.spectral_zero else self.fluoem_norm
+        p.spectral_psf_norm = float(numpy.sum(spectral) * self.psf_normalization)
         p.four_pi = float(4.0 * numpy.pi)
         p.focal_depth = float(self.detector_focal_point[0])
         p.depth_cutoff = float(self.depth_cutoff)
@@ -365,6 +373,8 @@ class _EPIFMSimulator:
             return
         p = numpy.ascontiguousarray(particles, dtype=numpy.float64)
         c = self.configs
+        if c.filter_error is not None:
+            raise TypeError(c.filter_error)
         keys = depth_keys(p[:, 0], c.detector_focal_point[0], c.depth_cutoff)
         self.tables.ensure(keys)
         dev = _device.upload(p, device=self.device)

@@ -34,6 +34,8 @@ class BatchedSimulator:
         self.device = _device.require_gpu()
         conf = config[method] if hasattr(config, '__getitem__') else config
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
+        if c.filter_error is not None:
+            raise TypeError(c.filter_error)
         if not c.uniform_adc():
             raise NotImplementedError('fixed-pattern ADC maps are not supported in batches')
         if c.detector_type not in ('EMCCD', 'CCD', 'CMOS'):

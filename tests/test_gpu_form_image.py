@@ -146,3 +146,26 @@ def test_ccd_and_cmos_philox_moments(gpu, kind):
         var = E + nvar
         assert abs(x.mean() - (E + nmean)) < 5 * numpy.sqrt(var / n), (kind, E, x.mean())
         assert abs(x.var() / var - 1.0) < 0.03, (kind, E, x.var(), var)
+
+
+def test_filter_switches_follow_the_reference(gpu):
+    """_epifm.py:1308-1313: an emission filter switched on without a type
+    zeroes every weight (background only, same frame as the oracle); a
+    dichroic mirror switched on with a catalog type raises TypeError."""
+    import scopyon_amd
+    pl = 16e-6 / 100.0
+    inputs = numpy.random.RandomState(2).uniform(-10 * pl, 10 * pl, size=(15, 2))
+    ov = {'detector': {'image_size': [24, 24], 'exposure_time': 0.033}, 'magnification': 100.0,
+          'emission_filter': {'switch': True}}
+    img, info = scopyon_amd.form_image(inputs, config=product_config(ov),
+                                       rng=numpy.random.RandomState(4), full_output=True)
+    (cam, _), = _oracle_frames(ov, inputs, 4)
+    assert numpy.array_equal(img.as_array(), cam[:, :, 1])
+    numpy.testing.assert_allclose(info['expectation'], cam[:, :, 0], rtol=1e-12)
+    assert numpy.allclose(cam[:, :, 0], cam[0, 0, 0])
+    ov = {'detector': {'image_size': [24, 24], 'exposure_time': 0.033}, 'magnification': 100.0,
+          'dichroic_mirror': {'switch': True}}
+    with pytest.raises(TypeError):
+        scopyon_amd.form_image(inputs, config=product_config(ov), rng=numpy.random.RandomState(4))
+    with pytest.raises(TypeError):
+        _oracle_frames(ov, inputs, 4)

@@ -96,3 +96,23 @@ def test_ccd_cmos_loops_equal_vectorised_draws():
     cdf /= cdf[-1]
     want = table[numpy.searchsorted(cdf, b.random_sample(63), side='right'), 0].reshape(7, 9)
     numpy.testing.assert_array_equal(got, want)
+
+
+def test_filter_switches_restate_the_reference_arithmetic():
+    """The reference scales fluoem_norm in place by 0.01 * efficiency
+    (_epifm.py:1308-1313): a catalog efficiency is a list (tolist(),
+    _epifm.py:996) and the product raises TypeError; no type gives zeros."""
+    from oracle import params as P
+    from conftest import oracle_params, PRE
+    p = oracle_params({'emission_filter': {'switch': True},
+                       'detector': {'image_size': [16, 16]}, 'magnification': 100.0})
+    assert isinstance(p.emission_eff, numpy.ndarray) and not p.emission_eff.any()
+    psf = O.PSFCache(p)
+    parts = numpy.array([[0.0, 1e-7, -2e-7, 0, 1.0]])
+    img, _, _ = O.molecule_plane(p, psf, parts, (16, 16), numpy.zeros(3), 0.033, {}, None, None)
+    assert not img.any()
+    p = oracle_params({'dichroic_mirror': {'switch': True},
+                       'detector': {'image_size': [16, 16]}, 'magnification': 100.0})
+    assert isinstance(p.dichroic_eff, list)
+    with pytest.raises(TypeError):
+        O.molecule_plane(p, O.PSFCache(p), parts, (16, 16), numpy.zeros(3), 0.033, {}, None, None)
