@@ -80,7 +80,7 @@ __device__ inline int frame_of(const int* __restrict__ frame_id, int i) { return
 __global__ void gauss_bin_count_kernel(int n, const float* __restrict__ x, const float* __restrict__ y,
                                        const int* __restrict__ frame_id,
                                        const smx_fov_t* __restrict__ frames, int nframes, int R,
-                                       int n_tiles, int* __restrict__ tile_count) {
+                                       int n_tiles, int* __restrict__ tile_count, int cs) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int f = frame_of(frame_id, i);
@@ -91,7 +91,7 @@ __global__ void gauss_bin_count_kernel(int n, const float* __restrict__ x, const
     for (int r = t.r0; r <= t.r1; ++r)
         for (int c = t.c0; c <= t.c1; ++c) {
             int tile = F.tile_off + r * F.tiles_c + c;
-            if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[tile], 1);
+            if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[(size_t)tile * cs], 1);
         }
 }
 
@@ -99,7 +99,7 @@ __global__ void gauss_bin_fill_kernel(int n, const float* __restrict__ x, const 
                                       const int* __restrict__ frame_id,
                                       const smx_fov_t* __restrict__ frames, int nframes, int R,
                                       int n_tiles, const int* __restrict__ tile_begin,
-                                      int* __restrict__ tile_left, int* __restrict__ list,
+                                      int* __restrict__ tile_left, int cs, int* __restrict__ list,
                                       long long capacity, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -112,7 +112,7 @@ __global__ void gauss_bin_fill_kernel(int n, const float* __restrict__ x, const 
         for (int c = t.c0; c <= t.c1; ++c) {
             int tile = F.tile_off + r * F.tiles_c + c;
             if (tile < 0 || tile >= n_tiles) continue;
-            int pos = atomicSub(&tile_left[tile], 1) - 1;   // counts run down to 0
+            int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;   // counts run down to 0
             long long at = (long long)tile_begin[tile] + pos;
             if (at < capacity) list[at] = i;
             else atomicOr(status, 2);
@@ -561,7 +561,8 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
 }
 
 struct GaussWorkspace {
-    int* tile_count;   // n_tiles + 1
+    int* tile_count;   // (n_tiles + 1) x COUNT_STRIDE (padded counters)
+    int* tile_dense;   // n_tiles + 1
     int* tile_begin;   // n_tiles + 1
     int* tile_fov;     // n_tiles
     int* list;         // capacity
@@ -575,7 +576,8 @@ struct GaussWorkspace {
 size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWorkspace* ws) {
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
-    size_t o_count = take(sizeof(int) * (n_tiles + 1));
+    size_t o_count = take(sizeof(int) * (size_t)(n_tiles + 1) * smx::COUNT_STRIDE);
+    size_t o_dense = take(sizeof(int) * (n_tiles + 1));
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
@@ -587,6 +589,7 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     size_t o_cub = take(cb);
     if (ws && base) {
         ws->tile_count = (int*)(base + o_count);
+        ws->tile_dense = (int*)(base + o_dense);
         ws->tile_begin = (int*)(base + o_begin);
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
@@ -624,23 +627,27 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
               const smx_fov_t* frames_dev, int nframes, int n_tiles, int R, long long capacity,
               GaussWorkspace& ws, int* status_dev, hipStream_t stream) {
     const int nb = (n + 255) / 256;
+    const int cs = smx::count_stride(2LL * n, n_tiles);
     hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                       stream, nframes, frames_dev, n_tiles, ws.tile_count, ws.tile_fov);
+                       stream, nframes, frames_dev, n_tiles, ws.tile_count, cs, ws.tile_fov);
     SMX_CHECK_LAUNCH("tiles_init_kernel");
     if (n > 0) {
         hipLaunchKernelGGL(gauss_bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, x, y,
-                           frame_id, frames_dev, nframes, R, n_tiles, ws.tile_count);
+                           frame_id, frames_dev, nframes, R, n_tiles, ws.tile_count, cs);
         SMX_CHECK_LAUNCH("gauss_bin_count_kernel");
     }
     size_t cb = ws.cub_bytes;
-    if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_count,
+    hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
+                       stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
+    SMX_CHECK_LAUNCH("compact_counts_kernel");
+    if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
                                                         ws.tile_begin, n_tiles + 1, stream),
                        "hipcub ExclusiveSum"))
         return SMX_ERUNTIME;
     if (n > 0) {
         hipLaunchKernelGGL(gauss_bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, x, y,
                            frame_id, frames_dev, nframes, R, n_tiles, ws.tile_begin,
-                           ws.tile_count, ws.list, capacity, status_dev);
+                           ws.tile_count, cs, ws.list, capacity, status_dev);
         SMX_CHECK_LAUNCH("gauss_bin_fill_kernel");
     }
     return SMX_OK;

@@ -124,7 +124,7 @@ __device__ inline bool fov_ok(int f, int nfov) { return f >= 0 && f < nfov; }
 __global__ void bin_count_kernel(int n, const double* __restrict__ u, const double* __restrict__ v,
                                  const double* __restrict__ w, const int* __restrict__ fov_id,
                                  const smx_fov_t* __restrict__ fovs, int nfov, double ext_half,
-                                 int n_tiles, int* __restrict__ tile_count) {
+                                 int n_tiles, int* __restrict__ tile_count, int cs) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (!(w[i] > 0.0)) return;
@@ -137,7 +137,7 @@ __global__ void bin_count_kernel(int n, const double* __restrict__ u, const doub
         for (int c = t.c0; c <= t.c1; ++c)
         {
             int tile = F.tile_off + r * F.tiles_c + c;
-            if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[tile], 1);
+            if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[(size_t)tile * cs], 1);
         }
 }
 
@@ -145,7 +145,7 @@ __global__ void bin_fill_kernel(int n, const double* __restrict__ u, const doubl
                                 const double* __restrict__ w, const int* __restrict__ fov_id,
                                 const smx_fov_t* __restrict__ fovs, int nfov, double ext_half,
                                 int n_tiles, const int* __restrict__ tile_begin,
-                                int* __restrict__ tile_left,
+                                int* __restrict__ tile_left, int cs,
                                 int* __restrict__ list, long long capacity, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -159,7 +159,7 @@ __global__ void bin_fill_kernel(int n, const double* __restrict__ u, const doubl
         for (int c = t.c0; c <= t.c1; ++c) {
             int tile = F.tile_off + r * F.tiles_c + c;
             if (tile < 0 || tile >= n_tiles) continue;
-            int pos = atomicSub(&tile_left[tile], 1) - 1;   // counts run down to 0
+            int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;   // counts run down to 0
             long long at = (long long)tile_begin[tile] + pos;
             if (at < capacity) list[at] = i;
             else atomicOr(status, 2);
@@ -415,7 +415,7 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
                                    int nfov, double ext_half, double res, int ntab, int P, int M,
                                    int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
                                    EmitterRec* __restrict__ rec, int n_tiles,
-                                   int* __restrict__ tile_count, int* __restrict__ status) {
+                                   int* __restrict__ tile_count, int cs, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     EmitterRec r;
@@ -489,7 +489,7 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
             for (int a = t.r0; a <= t.r1; ++a)
                 for (int b = t.c0; b <= t.c1; ++b) {
                     const int tile = F.tile_off + a * F.tiles_c + b;
-                    if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[tile], 1);
+                    if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[(size_t)tile * cs], 1);
                 }
     }
 }
@@ -500,7 +500,7 @@ __global__ void bin_fill_rec_kernel(int n, const EmitterRec* __restrict__ rec,
                                     const int* __restrict__ fov_id,
                                     const smx_fov_t* __restrict__ fovs, int nfov, int n_tiles,
                                     const int* __restrict__ tile_begin,
-                                    int* __restrict__ tile_left, int* __restrict__ list,
+                                    int* __restrict__ tile_left, int cs, int* __restrict__ list,
                                     long long capacity, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -515,7 +515,7 @@ __global__ void bin_fill_rec_kernel(int n, const EmitterRec* __restrict__ rec,
         for (int b = t.c0; b <= t.c1; ++b) {
             const int tile = F.tile_off + a * F.tiles_c + b;
             if (tile < 0 || tile >= n_tiles) continue;
-            const int pos = atomicSub(&tile_left[tile], 1) - 1;   // counts run down to 0
+            const int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;   // counts run down to 0
             const long long at = (long long)tile_begin[tile] + pos;
             if (at < capacity) list[at] = i;
             else atomicOr(status, 2);
@@ -820,7 +820,8 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
 
 struct SplatWorkspace {
     EmitterRec* rec;   // n (phase-table path)
-    int* tile_count;   // n_tiles + 1
+    int* tile_count;   // (n_tiles + 1) x COUNT_STRIDE (padded counters)
+    int* tile_dense;   // n_tiles + 1
     int* tile_begin;   // n_tiles + 1
     int* tile_fov;     // n_tiles
     int* list;         // capacity
@@ -838,7 +839,8 @@ size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, Spla
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
     size_t o_rec = take(sizeof(EmitterRec) * (size_t)(n > 0 ? n : 1));
-    size_t o_count = take(sizeof(int) * (n_tiles + 1));
+    size_t o_count = take(sizeof(int) * (size_t)(n_tiles + 1) * smx::COUNT_STRIDE);
+    size_t o_dense = take(sizeof(int) * (n_tiles + 1));
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
@@ -847,6 +849,7 @@ size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, Spla
     if (ws && base) {
         ws->rec = (EmitterRec*)(base + o_rec);
         ws->tile_count = (int*)(base + o_count);
+        ws->tile_dense = (int*)(base + o_dense);
         ws->tile_begin = (int*)(base + o_begin);
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
@@ -938,10 +941,12 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     layout_workspace(n, n_tiles, list_capacity, (char*)workspace, &ws);
     const double ext_half = (res * (double)(ntab - 1)) * 0.5;   // signal_width * 0.5
     const int nb = (n + 255) / 256;
+    // about two tiles per emitter at these footprints
+    const int cs = smx::count_stride(2LL * n, n_tiles);
 
     if (stages & 1) {
         hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                           stream, nfov, fovs_dev, n_tiles, ws.tile_count, ws.tile_fov);
+                           stream, nfov, fovs_dev, n_tiles, ws.tile_count, cs, ws.tile_fov);
         SMX_CHECK_LAUNCH("tiles_init_kernel");
         if (n > 0) {
             if (K) {
@@ -951,16 +956,19 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                                    key, fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m,
                                    n_slots,
                                    folded ? PF.elems() : smx_phase_table_elems(sat_period, m),
-                                   folded, PF, ws.rec, n_tiles, ws.tile_count, status_dev);
+                                   folded, PF, ws.rec, n_tiles, ws.tile_count, cs, status_dev);
                 SMX_CHECK_LAUNCH("emitter_rec_kernel");
             } else {
                 hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
-                                   fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_count);
+                                   fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_count, cs);
                 SMX_CHECK_LAUNCH("bin_count_kernel");
             }
         }
         size_t cb = ws.cub_bytes;
-        if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_count,
+        hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
+                           stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
+        SMX_CHECK_LAUNCH("compact_counts_kernel");
+        if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
                                                             ws.tile_begin, n_tiles + 1, stream),
                            "hipcub ExclusiveSum"))
             return SMX_ERUNTIME;
@@ -968,11 +976,11 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
             if (K)
                 hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nb), dim3(256), 0, stream, n, ws.rec,
                                    fov_id, fovs_dev, nfov, n_tiles, ws.tile_begin,
-                                   ws.tile_count, ws.list, list_capacity, status_dev);
+                                   ws.tile_count, cs, ws.list, list_capacity, status_dev);
             else
                 hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
                                    fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin,
-                                   ws.tile_count, ws.list, list_capacity, status_dev);
+                                   ws.tile_count, cs, ws.list, list_capacity, status_dev);
             SMX_CHECK_LAUNCH("bin_fill_kernel");
         }
     }

@@ -28,15 +28,30 @@ __global__ void tile_fov_kernel(int nfov, const smx_fov_t* __restrict__ fovs,
     }
 }
 
-// One pass before binning: tile_count[0..n_tiles] = 0 and tile_fov[g] = the
-// frame owning global tile g (-1 where none does).  The frames' tile ranges
+// Per-tile counters are spaced `cs` ints apart (1, or COUNT_STRIDE = one
+// 128-byte line): with many emitters per tile the binning atomics serialise
+// on shared lines, with few the padding only spreads the working set.
+constexpr int COUNT_STRIDE = 32;
+inline int count_stride(long long entries, int n_tiles) {
+    return entries >= 128LL * n_tiles ? COUNT_STRIDE : 1;
+}
+
+// Dense copy of the counters for the scan: dense[g] = counts[g * cs].
+__global__ void compact_counts_kernel(const int* __restrict__ counts, int cs, int n,
+                                      int* __restrict__ dense) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) dense[g] = counts[(size_t)g * cs];
+}
+
+// One pass before binning: tile_count[g * cs] = 0 for g <= n_tiles and
+// tile_fov[g] = the frame owning global tile g (-1 where none does).  The frames' tile ranges
 // are staged in LDS (up to TILES_INIT_FOVS of them) and searched by bisection
 // when their offsets ascend, linearly otherwise.
 constexpr int TILES_INIT_FOVS = 2048;
 __global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
                                                          const smx_fov_t* __restrict__ fovs,
                                                          int n_tiles, int* __restrict__ tile_count,
-                                                         int* __restrict__ tile_fov) {
+                                                         int cs, int* __restrict__ tile_fov) {
     __shared__ int s_off[TILES_INIT_FOVS], s_end[TILES_INIT_FOVS];
     __shared__ int s_unsorted;
     const int staged = nfov < TILES_INIT_FOVS ? nfov : TILES_INIT_FOVS;
@@ -52,7 +67,7 @@ __global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
     __syncthreads();
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g > n_tiles) return;
-    tile_count[g] = 0;
+    tile_count[(size_t)g * cs] = 0;
     if (g == n_tiles) return;
     int owner = -1;
     if (!s_unsorted && staged == nfov) {
