@@ -503,21 +503,35 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
     const int W = A.W;
-    for (int base = 0; base < count; base += 16) {
+    // factors of chunk `base` (16 emitters x 32 floats, 8 per lane) are
+    // fetched into registers one chunk ahead, so their latency overlaps the
+    // previous chunk's fragment build and MFMA
+    float pre[8];
+    int2 pre_lo = make_int2(0, 0);   // lanes 0..15: window origin of emitter `l`
+    auto fetch = [&](int base) {
         const int nk = min(16, count - base);
-        // stage the chunk's factors: 16 emitters x 32 floats, 8 per lane
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int idx = r * 64 + l, k = idx >> 5, t = idx & 31;
-            float v = 0.0f;
+            pre[r] = 0.0f;
             if (k < nk) {
                 const int e = sorted ? s_sort[base + k] : list[base + k];
-                v = A.fac[(size_t)e * 32 + t];
-                if (t == 0) s_lo[k] = A.lo[e];
+                pre[r] = A.fac[(size_t)e * 32 + t];
             }
-            s_fac[k][t] = v;
         }
+        if (l < nk) pre_lo = A.lo[sorted ? s_sort[base + l] : list[base + l]];
+    };
+    if (count > 0) fetch(0);
+    for (int base = 0; base < count; base += 16) {
+        const int nk = min(16, count - base);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int idx = r * 64 + l, k = idx >> 5, t = idx & 31;
+            s_fac[k][t] = pre[r];
+        }
+        if (l < nk) s_lo[l] = pre_lo;
         __syncthreads();
+        if (base + 16 < count) fetch(base + 16);
         if (DT == 0) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
