@@ -508,18 +508,23 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
     // previous chunk's fragment build and MFMA
     float pre[8];
     int2 pre_lo = make_int2(0, 0);   // lanes 0..15: window origin of emitter `l`
+    // branch-free: indices clamped to the list, out-of-chunk values masked,
+    // so the eight loads issue back to back
     auto fetch = [&](int base) {
-        const int nk = min(16, count - base);
+        const int last = count - 1;
+        int e[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int idx = r * 64 + l, k = idx >> 5, t = idx & 31;
-            pre[r] = 0.0f;
-            if (k < nk) {
-                const int e = sorted ? s_sort[base + k] : list[base + k];
-                pre[r] = A.fac[(size_t)e * 32 + t];
-            }
+            const int k = 2 * r + (l >> 5);
+            e[r] = sorted ? s_sort[min(base + k, last)] : list[min(base + k, last)];
         }
-        if (l < nk) pre_lo = A.lo[sorted ? s_sort[base + l] : list[base + l]];
+        const int el = sorted ? s_sort[min(base + (l & 15), last)] : list[min(base + (l & 15), last)];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) pre[r] = A.fac[(size_t)e[r] * 32 + (l & 31)];
+        pre_lo = A.lo[el];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            if (base + 2 * r + (l >> 5) > last) pre[r] = 0.0f;
     };
     if (count > 0) fetch(0);
     for (int base = 0; base < count; base += 16) {
@@ -532,31 +537,39 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
         if (l < nk) s_lo[l] = pre_lo;
         __syncthreads();
         if (base + 16 < count) fetch(base + 16);
+        // operands: window-origin reads first, then clamped factor reads,
+        // masked outside the window (no branches between the LDS reads)
         if (DT == 0) {
+            int2 o[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[j] = s_lo[2 * j + half];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int k = 2 * j + half;
-                const int2 o = s_lo[k];
-                const int ru = row - o.x, cv = col - o.y;
+                const int ru = row - o[j].x, cv = col - o[j].y;
                 const bool ok = k < nk;
-                const float a = (ok && (unsigned)ru < (unsigned)W) ? s_fac[k][ru] : 0.0f;
-                const float b = (ok && (unsigned)cv < (unsigned)W) ? s_fac[k][16 + cv] : 0.0f;
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+                const float a = s_fac[k][min(max(ru, 0), 15)];
+                const float b = s_fac[k][16 + min(max(cv, 0), 15)];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                    (ok && (unsigned)ru < (unsigned)W) ? a : 0.0f,
+                    (ok && (unsigned)cv < (unsigned)W) ? b : 0.0f, acc, 0, 0, 0);
             }
         } else {
             typedef typename std::conditional<DT == 1, _Float16, __bf16>::type E;
             typedef typename Frag8<E>::type V8;
             V8 a, b;
+            int2 o[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) o[t] = s_lo[half * 8 + t];
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const int k = half * 8 + t;
-                const int2 o = s_lo[k];
-                const int ru = row - o.x, cv = col - o.y;
+                const int ru = row - o[t].x, cv = col - o[t].y;
                 const bool ok = k < nk;
-                const float av = (ok && (unsigned)ru < (unsigned)W) ? s_fac[k][ru] : 0.0f;
-                const float bv = (ok && (unsigned)cv < (unsigned)W) ? s_fac[k][16 + cv] : 0.0f;
-                a[t] = (E)av;
-                b[t] = (E)bv;
+                const float av = s_fac[k][min(max(ru, 0), 15)];
+                const float bv = s_fac[k][16 + min(max(cv, 0), 15)];
+                a[t] = (E)((ok && (unsigned)ru < (unsigned)W) ? av : 0.0f);
+                b[t] = (E)((ok && (unsigned)cv < (unsigned)W) ? bv : 0.0f);
             }
             if constexpr (DT == 1) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
             else acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
