@@ -32,6 +32,8 @@ using smx::align256;
 using smx::fill_int_kernel;
 using smx::lds_add;
 using smx::lds_bitonic_sort;
+using smx::wave_sort_list;
+using smx::WAVE_SORT_MAX;
 using smx::tile_fov_kernel;
 
 constexpr int TS = 32;           // tile side
@@ -165,12 +167,15 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && sorted) {
+    if (count >= 1 && count <= WAVE_SORT_MAX) {
+        if (wave == 0) wave_sort_list(list, count, lane, s_sort);
+        __syncthreads();
+    } else if (count >= 1 && sorted) {
         int np2 = 1;
         while (np2 < count) np2 <<= 1;
         for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
         __syncthreads();
-        if (count > 1) lds_bitonic_sort(s_sort, np2);
+        lds_bitonic_sort(s_sort, np2);
     } else if (!sorted && threadIdx.x == 0) {
         atomicOr(A.status, 1);
     }
@@ -490,12 +495,15 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && sorted) {
+    if (count >= 1 && count <= WAVE_SORT_MAX) {
+        wave_sort_list(list, count, l, s_sort);
+        __syncthreads();
+    } else if (count >= 1 && sorted) {
         int np2 = 1;
         while (np2 < count) np2 <<= 1;
         for (int k = l; k < np2; k += 64) s_sort[k] = k < count ? list[k] : 0x7fffffff;
         __syncthreads();
-        if (count > 1) lds_bitonic_sort(s_sort, np2);
+        lds_bitonic_sort(s_sort, np2);
     } else if (!sorted && l == 0) {
         atomicOr(A.status, 1);
     }

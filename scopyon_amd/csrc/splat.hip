@@ -28,6 +28,8 @@ using smx::align256;
 using smx::fill_int_kernel;
 using smx::lds_add;
 using smx::lds_bitonic_sort;
+using smx::wave_sort_list;
+using smx::WAVE_SORT_MAX;
 using smx::tile_fov_kernel;
 
 constexpr int TS = 32;         // tile side in pixels
@@ -630,32 +632,6 @@ constexpr int QS = 16;            // quadrant side
 constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
 constexpr int PSORT_CAP = 2048;   // tile list sorted in LDS
 
-// Ascending bitonic sort of 128 ints held by one wave, element `lane` in a0
-// and element `lane + 64` in a1 (pad with INT_MAX); written to out[0..127].
-// Stage (k, j) pairs element p with p ^ j; the lower of the pair keeps the
-// minimum when p & k == 0 (ascending block), the maximum otherwise.
-__device__ inline void wave_sort128(int a0, int a1, int lane, int* out) {
-#pragma unroll
-    for (int k = 2; k <= 128; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j == 64) {           // partner of element lane is lane + 64 (k == 128)
-                const int lo = min(a0, a1), hi = max(a0, a1);
-                a0 = lo;
-                a1 = hi;
-            } else {
-                const int b0 = __shfl_xor(a0, j), b1 = __shfl_xor(a1, j);
-                const bool lower = (lane & j) == 0;
-                const bool up0 = (lane & k) == 0, up1 = ((lane + 64) & k) == 0;
-                a0 = (lower == up0) ? min(a0, b0) : max(a0, b0);
-                a1 = (lower == up1) ? min(a1, b1) : max(a1, b1);
-            }
-        }
-    }
-    out[lane] = a0;
-    out[lane + 64] = a1;
-}
-
 template <typename KT>
 __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     __shared__ int s_sort[PSORT_CAP];
@@ -687,12 +663,9 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && count <= 128) {
-        // short list: one wave sorts it in registers (two entries per lane)
-        if (wave == 0) {
-            wave_sort128(lane < count ? list[lane] : 0x7fffffff,
-                         lane + 64 < count ? list[lane + 64] : 0x7fffffff, lane, s_sort);
-        }
+    if (count >= 1 && count <= WAVE_SORT_MAX) {
+        // list of up to 1024 entries: one wave sorts it in registers
+        if (wave == 0) wave_sort_list(list, count, lane, s_sort);
         __syncthreads();
     } else if (count >= 1 && sorted) {
         int np2 = 1;

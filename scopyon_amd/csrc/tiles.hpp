@@ -111,6 +111,60 @@ __device__ inline void lds_bitonic_sort(int* a, int n_pow2) {
     }
 }
 
+// Ascending bitonic sort of up to 64 * NPL ints by one wave, in registers:
+// element p = r * 64 + lane lives in v[r] (entries past `count` padded with
+// INT_MAX).  Stage (k, j) pairs p with p ^ j -- across lanes for j < 64 (one
+// shuffle), inside the lane for j >= 64 -- and the lower of the pair keeps
+// the minimum when p & k == 0, the maximum otherwise.  The sorted list is
+// written to out[0 .. 64 * NPL).
+template <int NPL>
+__device__ inline void wave_sort(const int* src, int count, int lane, int* out) {
+    int v[NPL];
+#pragma unroll
+    for (int r = 0; r < NPL; ++r) {
+        const int p = r * 64 + lane;
+        v[r] = p < count ? src[p] : 0x7fffffff;
+    }
+#pragma unroll
+    for (int k = 2; k <= 64 * NPL; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int m = j >> 6;
+#pragma unroll
+                for (int r = 0; r < NPL; ++r) {
+                    if (r & m) continue;
+                    const bool up = ((r * 64 + lane) & k) == 0;
+                    const int lo = min(v[r], v[r | m]), hi = max(v[r], v[r | m]);
+                    v[r] = up ? lo : hi;
+                    v[r | m] = up ? hi : lo;
+                }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int r = 0; r < NPL; ++r) {
+                    const int b = __shfl_xor(v[r], j);
+                    const bool up = ((r * 64 + lane) & k) == 0;
+                    v[r] = (lower == up) ? min(v[r], b) : max(v[r], b);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NPL; ++r) out[r * 64 + lane] = v[r];
+}
+
+// Sort a tile list of `count` <= WAVE_SORT_MAX entries into out[] by one
+// wave, in registers; out[] must hold the count rounded up to 128, 256, 512
+// or 1024 (the tail is INT_MAX).
+constexpr int WAVE_SORT_MAX = 1024;
+__device__ inline void wave_sort_list(const int* src, int count, int lane, int* out) {
+    if (count <= 128) wave_sort<2>(src, count, lane, out);
+    else if (count <= 256) wave_sort<4>(src, count, lane, out);
+    else if (count <= 512) wave_sort<8>(src, count, lane, out);
+    else wave_sort<16>(src, count, lane, out);
+}
+
 __device__ inline void lds_add(double* p, double v) {
     // ds_add_f64: one IEEE add in LDS, in program order per wave
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
