@@ -228,66 +228,224 @@ __device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk, bool f
     return d;
 }
 
-template <typename OutT>
-__global__ void emccd_philox_kernel(int n, const double* __restrict__ photons,
-                                    const double* __restrict__ gain_map,
-                                    const double* __restrict__ offset_map, smx_detector_params_t P,
-                                    uint64_t seed, uint32_t frame, uint32_t pix_base,
-                                    const uint32_t* __restrict__ frame_dev,
-                                    double* __restrict__ E_out, OutT* __restrict__ adc_out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (frame_dev) frame += *frame_dev;
-    // float32 expectation, noise and ADC (a statistical path, not a parity path)
-    float ph = (float)photons[i];
-    if (P.background_switch) ph += (float)P.background;
-    const float E = (float)P.qe * ph;
-    PixelRng R{pix_base + (uint32_t)i, frame, (uint32_t)seed, (uint32_t)(seed >> 32)};
-    const smx::u32x4 r0 = R.block(0);
-    // block 0: (x, y) -> readout-noise normal and a spare normal for the first
-    // Gamma proposal, z -> Poisson uniform, w -> first Gamma uniform
-    const float2 nn = normal2_f(r0.x, r0.y);
-    float noise = 0.0f;
-    if (P.readout_noise > 0.0) noise = (float)P.readout_noise * nn.x;
-    float S = 0.0f;
-    if (E > 0.0f) {
-        const float lam = E;
-        int k = 0;
-        if (lam < 12.0f) {
-            // inversion: smallest k with u < P(X <= k)
-            float u = u01f(r0.z);
-            float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * lam), cum = p;
-            while (u > cum && k < 4096) {
-                ++k;
-                p *= lam * __builtin_amdgcn_rcpf((float)k);
-                cum += p;
-            }
-        } else {
-            k = poisson_ptrs(R, lam);
-        }
-        if (k > 0) {
-            // EM register: S = round(Gamma(k, g)), redrawn below 0.5 (S >= 1)
-            // Marsaglia-Tsang for every shape k >= 1 (one code path for the
-            // whole wave: no divergence between k = 1 and k > 1)
-            uint32_t blk = 1u << 16;
-            float s = 0.0f;
-            for (int guard = 0; guard < 64; ++guard) {
-                s = gamma_mt(R, k, blk, guard == 0, nn.y, u01f_open0(r0.w)) * (float)P.emgain;
-                if (s >= 0.5f) break;
-            }
-            S = floorf(s + 0.5f);
-            if (S < 1.0f) S = 1.0f;
-        }
+// EMCCD (_epifm.py:1448-1454, noise and register of _epifm.py:355-416) on
+// Philox streams, float32.  Per pixel: block 0 gives the readout normal, a
+// spare normal and a spare uniform for the first Gamma proposal, and the
+// Poisson uniform; k = Poisson(E) by inversion below 12, else PTRS on blocks
+// 1..; the EM register S = round(Gamma(k, emgain)) by Marsaglia-Tsang (every
+// shape k >= 1 on one code path), redrawn below 0.5, on blocks 2^16..; then
+// ADC.  Every draw comes from a fixed block of the pixel's own stream, so the
+// result does not depend on which thread computes it.
+//
+// Work queues: a block of 256 threads owns 1024 pixels (4 per thread,
+// coalesced).  With one thread per pixel a wave would run the inversion loop,
+// PTRS and every Marsaglia-Tsang retry whenever any of its 64 lanes needs
+// them; here the pixels that take PTRS (lam >= 12) and those with k >= 1 are
+// appended to LDS queues and drained by the whole block, and a Gamma
+// proposal rejected on the first (block-0 spare) draw goes to a retry queue,
+// so a wave runs a retry only for pixels that need one.
+constexpr int EQ_THREADS = 256, EQ_PPT = 4, EQ_PIX = EQ_THREADS * EQ_PPT;
+
+// Wave-aggregated append to an LDS queue: one LDS atomic per wave; returns
+// the slot of an appending lane (-1 for the others).  Call with the wave's
+// active lanes converged.
+__device__ inline int queue_push(bool pred, int* counter) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return -1;
+    const int leader = __ffsll((long long)m) - 1;
+    const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    return pred ? base + rank : -1;
+}
+
+// Marsaglia-Tsang iterations it0, it0 + 1, ... of Gamma(n, 1) on Philox
+// blocks blk, blk + 1, ... (the loop of gamma_mt without the first-draw
+// shortcut).
+__device__ inline float gamma_mt_from(const PixelRng& R, int n, uint32_t& blk, int it0) {
+    const float d = (float)n - 1.0f / 3.0f;
+    const float c = 1.0f / sqrtf(9.0f * d);
+    for (int it = it0; it < 256; ++it) {
+        smx::u32x4 r = R.block(blk++);
+        const float x = normal_f(r.x, r.y);
+        const float u = u01f_open0(r.z);
+        float v = 1.0f + c * x;
+        if (v <= 0.0f) continue;
+        v = v * v * v;
+        if (u < 1.0f - 0.0331f * (x * x) * (x * x)) return d * v;
+        if (ln_hw(u) < 0.5f * x * x + d * (1.0f - v + ln_hw(v))) return d * v;
     }
-    // ADC (_epifm.py:1475-1484): clip at full well, / gain, + offset, clip
-    const float gain = (float)(gain_map ? gain_map[i] : P.gain);
-    const float offset = (float)(offset_map ? offset_map[i] : P.offset);
-    float x = S + noise;
-    x = fminf(x, (float)P.fullwell);
-    x = x * __builtin_amdgcn_rcpf(gain) + offset;
-    x = fminf(fmaxf(x, 0.0f), (float)P.adc_max);
-    if (E_out) E_out[i] = (double)E;
-    adc_out[i] = (OutT)x;
+    return d;
+}
+
+template <typename OutT>
+__global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
+        int n, const double* __restrict__ photons, const double* __restrict__ gain_map,
+        const double* __restrict__ offset_map, smx_detector_params_t P, uint64_t seed,
+        uint32_t frame, uint32_t pix_base, const uint32_t* __restrict__ frame_dev,
+        double* __restrict__ E_out, OutT* __restrict__ adc_out) {
+    __shared__ int s_k[EQ_PIX];          // Poisson count
+    __shared__ float s_lam[EQ_PIX];      // expectation (PTRS items)
+    __shared__ float s_x0[EQ_PIX];       // spare normal of block 0
+    __shared__ float s_u0[EQ_PIX];       // spare uniform of block 0
+    __shared__ float s_S[EQ_PIX];        // EM-register output
+    __shared__ short s_q[EQ_PIX];        // PTRS queue, then the Gamma queue
+    __shared__ short s_r[EQ_PIX];        // Gamma retry queue (first draw rejected: flag 1 << 14)
+    __shared__ int s_n[3];
+
+    const int t = threadIdx.x;
+    const long long base = (long long)blockIdx.x * EQ_PIX;
+    if (t < 3) s_n[t] = 0;
+    if (frame_dev) frame += *frame_dev;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const float emgain = (float)P.emgain;
+    __syncthreads();
+
+    // phase 1: block 0 of every pixel (readout normal, spare normal, Poisson
+    // inversion below 12, spare uniform); PTRS pixels queued
+    float noise[EQ_PPT];
+#pragma unroll
+    for (int j = 0; j < EQ_PPT; ++j) {
+        const int loc = t + j * EQ_THREADS;
+        const long long i = base + loc;
+        const bool in = i < n;
+        noise[j] = 0.0f;
+        int k = 0;
+        float E = 0.0f, x0 = 0.0f, u0 = 1.0f;
+        bool ptrs = false;
+        if (in) {
+            float ph = (float)photons[i];
+            if (P.background_switch) ph += (float)P.background;
+            E = (float)P.qe * ph;
+            const PixelRng R{pix_base + (uint32_t)i, frame, k0, k1};
+            const smx::u32x4 r0 = R.block(0);
+            const float2 nn = normal2_f(r0.x, r0.y);
+            if (P.readout_noise > 0.0) noise[j] = (float)P.readout_noise * nn.x;
+            x0 = nn.y;
+            u0 = u01f_open0(r0.w);
+            if (E > 0.0f) {
+                if (E < 12.0f) {
+                    const float u = u01f(r0.z);
+                    float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * E), cum = p;
+                    while (u > cum && k < 4096) {
+                        ++k;
+                        p *= E * __builtin_amdgcn_rcpf((float)k);
+                        cum += p;
+                    }
+                } else {
+                    ptrs = true;
+                }
+            }
+            if (E_out) E_out[i] = (double)E;
+        }
+        s_k[loc] = k;
+        s_lam[loc] = E;
+        s_x0[loc] = x0;
+        s_u0[loc] = u0;
+        const int slot = queue_push(ptrs, &s_n[0]);
+        if (ptrs) s_q[slot] = (short)loc;
+    }
+    __syncthreads();
+
+    // phase 2: PTRS (lam >= 12), one queued pixel per lane
+    const int np = s_n[0];
+    for (int q = t; q < np; q += EQ_THREADS) {
+        const int loc = s_q[q];
+        const PixelRng R{pix_base + (uint32_t)(base + loc), frame, k0, k1};
+        s_k[loc] = poisson_ptrs(R, s_lam[loc]);
+    }
+    __syncthreads();
+
+    // phase 3: Gamma queue (k >= 1)
+#pragma unroll
+    for (int j = 0; j < EQ_PPT; ++j) {
+        const int loc = t + j * EQ_THREADS;
+        const bool g = s_k[loc] > 0;
+        const int slot = queue_push(g, &s_n[1]);
+        if (g) s_q[slot] = (short)loc;
+    }
+    __syncthreads();
+
+    // phase 4: first Marsaglia-Tsang proposal from the block-0 spares; a
+    // rejected proposal or a result below 0.5 goes to the retry queue
+    const int ng = s_n[1];
+    for (int q0 = 0; q0 < ng; q0 += EQ_THREADS) {
+        const int q = q0 + t;
+        bool retry = false;
+        int loc = 0;
+        bool rejected = false;
+        if (q < ng) {
+            loc = s_q[q];
+            const int k = s_k[loc];
+            const float d = (float)k - 1.0f / 3.0f;
+            const float c = 1.0f / sqrtf(9.0f * d);
+            const float x = s_x0[loc], u = s_u0[loc];
+            float v = 1.0f + c * x;
+            bool acc = false;
+            if (v > 0.0f) {
+                v = v * v * v;
+                acc = (u < 1.0f - 0.0331f * (x * x) * (x * x)) ||
+                      (ln_hw(u) < 0.5f * x * x + d * (1.0f - v + ln_hw(v)));
+            }
+            const float sv = d * v * emgain;
+            rejected = !acc;
+            if (acc && sv >= 0.5f) {
+                float S = floorf(sv + 0.5f);
+                if (S < 1.0f) S = 1.0f;
+                s_S[loc] = S;
+            } else {
+                retry = true;
+            }
+        }
+        const int slot = queue_push(retry, &s_n[2]);
+        if (retry) s_r[slot] = (short)(loc | (rejected ? (1 << 14) : 0));
+    }
+    __syncthreads();
+
+    // phase 5: retries, continuing the pixel's stream at block 2^16
+    const int nr = s_n[2];
+    for (int q = t; q < nr; q += EQ_THREADS) {
+        const int ent = s_r[q];
+        const int loc = ent & ((1 << 14) - 1);
+        const int k = s_k[loc];
+        const PixelRng R{pix_base + (uint32_t)(base + loc), frame, k0, k1};
+        uint32_t blk = 1u << 16;
+        float s;
+        if (ent & (1 << 14)) {
+            // first draw rejected: its Gamma call goes on with iteration 1
+            s = gamma_mt_from(R, k, blk, 1) * emgain;
+        } else {
+            const float d = (float)k - 1.0f / 3.0f;
+            const float c = 1.0f / sqrtf(9.0f * d);
+            const float v = 1.0f + c * s_x0[loc];
+            s = d * (v * v * v) * emgain;   // accepted, below 0.5
+        }
+        for (int guard = 1; guard < 64 && !(s >= 0.5f); ++guard)
+            s = gamma_mt_from(R, k, blk, 0) * emgain;
+        float S = floorf(s + 0.5f);
+        if (S < 1.0f) S = 1.0f;
+        s_S[loc] = S;
+    }
+    __syncthreads();
+
+    // phase 6: readout noise and ADC (_epifm.py:1475-1484)
+#pragma unroll
+    for (int j = 0; j < EQ_PPT; ++j) {
+        const int loc = t + j * EQ_THREADS;
+        const long long i = base + loc;
+        if (i >= n) continue;
+        const float S = s_k[loc] > 0 ? s_S[loc] : 0.0f;
+        const float gain = (float)(gain_map ? gain_map[i] : P.gain);
+        const float offset = (float)(offset_map ? offset_map[i] : P.offset);
+        float x = S + noise[j];
+        x = fminf(x, (float)P.fullwell);
+        x = x * __builtin_amdgcn_rcpf(gain) + offset;
+        x = fminf(fmaxf(x, 0.0f), (float)P.adc_max);
+        adc_out[i] = (OutT)x;
+    }
 }
 
 // searchsorted(cdf, u, side='right') clamped to the last index: the draw of
@@ -444,15 +602,16 @@ int smx_emccd_philox(int n, const double* photons, const double* gain_map,
         return SMX_EVALUE;
     }
     if (n == 0) return SMX_OK;
+    const dim3 grid((unsigned)((n + EQ_PIX - 1) / EQ_PIX));
     if (adc_is_float32)
-        hipLaunchKernelGGL(emccd_philox_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, stream,
-                           n, photons, gain_map, offset_map, *params, seed, frame, pix_base,
+        hipLaunchKernelGGL(emccd_queue_kernel<float>, grid, dim3(EQ_THREADS), 0, stream, n,
+                           photons, gain_map, offset_map, *params, seed, frame, pix_base,
                            frame_dev, E_out, (float*)adc_out);
     else
-        hipLaunchKernelGGL(emccd_philox_kernel<double>, dim3((n + 255) / 256), dim3(256), 0,
-                           stream, n, photons, gain_map, offset_map, *params, seed, frame,
-                           pix_base, frame_dev, E_out, (double*)adc_out);
-    SMX_CHECK_LAUNCH("emccd_philox_kernel");
+        hipLaunchKernelGGL(emccd_queue_kernel<double>, grid, dim3(EQ_THREADS), 0, stream, n,
+                           photons, gain_map, offset_map, *params, seed, frame, pix_base,
+                           frame_dev, E_out, (double*)adc_out);
+    SMX_CHECK_LAUNCH("emccd_queue_kernel");
     return SMX_OK;
 }
 
