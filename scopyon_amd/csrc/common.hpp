@@ -123,6 +123,21 @@ SMX_HD double np_remainder(double a, double b) {
 // Ceil of a double, returned as a 64-bit integer (numpy.ceil(..).astype(int)).
 SMX_HD long long ceil_i64(double x) { return (long long)ceil(x); }
 
+// ceil / floor of the correctly rounded float64 quotient num / den, from the
+// product q = num * inv with inv = 1 / den rounded.  q lies within 3.4e-16 |q|
+// of that quotient, so the two have the same ceiling and floor unless q is
+// within 1e-12 (relative) of an integer; there the division decides.
+SMX_HD long long ceil_div(double num, double den, double inv) {
+    const double q = num * inv;
+    if (fabs(q - rint(q)) > 1e-12 * fmax(1.0, fabs(q))) return (long long)ceil(q);
+    return (long long)ceil(num / den);
+}
+SMX_HD long long floor_div(double num, double den, double inv) {
+    const double q = num * inv;
+    if (fabs(q - rint(q)) > 1e-12 * fmax(1.0, fabs(q))) return (long long)floor(q);
+    return (long long)floor(num / den);
+}
+
 // Read through a pointer known to address global memory.  Pointers fetched
 // from by-value argument structs are otherwise treated as generic (flat)
 // pointers, and flat loads share the LDS wait counter, so every LDS wait

@@ -246,6 +246,18 @@ __device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk, bool f
 // so a wave runs a retry only for pixels that need one.
 constexpr int EQ_THREADS = 256, EQ_PPT = 4, EQ_PIX = EQ_THREADS * EQ_PPT;
 
+// Poisson(lam) by inversion: the smallest k with u < P(X <= k) (lam < 12).
+__device__ inline int poisson_inversion(float lam, float u) {
+    int k = 0;
+    float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * lam), cum = p;
+    while (u > cum && k < 4096) {
+        ++k;
+        p *= lam * __builtin_amdgcn_rcpf((float)k);
+        cum += p;
+    }
+    return k;
+}
+
 // Wave-aggregated append to an LDS queue: one LDS atomic per wave; returns
 // the slot of an appending lane (-1 for the others).  Call with the wave's
 // active lanes converged.
@@ -294,7 +306,7 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
     __shared__ float s_S[EQ_PIX];        // EM-register output
     __shared__ short s_q[EQ_PIX];        // PTRS queue, then the Gamma queue
     __shared__ short s_r[EQ_PIX];        // Gamma retry queue (first draw rejected: flag 1 << 14)
-    __shared__ int s_n[3];
+    __shared__ int s_n[3];               // PTRS, Gamma and retry queue lengths
 
     const int t = threadIdx.x;
     const long long base = (long long)blockIdx.x * EQ_PIX;
@@ -327,17 +339,8 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
             x0 = nn.y;
             u0 = u01f_open0(r0.w);
             if (E > 0.0f) {
-                if (E < 12.0f) {
-                    const float u = u01f(r0.z);
-                    float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * E), cum = p;
-                    while (u > cum && k < 4096) {
-                        ++k;
-                        p *= E * __builtin_amdgcn_rcpf((float)k);
-                        cum += p;
-                    }
-                } else {
-                    ptrs = true;
-                }
+                if (E < 12.0f) k = poisson_inversion(E, u01f(r0.z));
+                else ptrs = true;
             }
             if (E_out) E_out[i] = (double)E;
         }
@@ -500,19 +503,7 @@ __global__ void photon_philox_kernel(int n, const double* __restrict__ photons,
     if (tn > 0) noise = (float)tvals[table_index(tcdf, tn, smx::u01_53(r0.x, r0.y))];
     else if (P.readout_noise > 0.0) noise = (float)P.readout_noise * normal_f(r0.x, r0.y);
     int k = 0;
-    if (E > 0.0f) {
-        if (E < 12.0f) {
-            float u = u01f(r0.z);
-            float p = __builtin_amdgcn_exp2f(-1.4426950408889634f * E), cum = p;
-            while (u > cum && k < 4096) {
-                ++k;
-                p *= E * __builtin_amdgcn_rcpf((float)k);
-                cum += p;
-            }
-        } else {
-            k = poisson_ptrs(R, E);
-        }
-    }
+    if (E > 0.0f) k = E < 12.0f ? poisson_inversion(E, u01f(r0.z)) : poisson_ptrs(R, E);
     const float gain = (float)(gain_map ? gain_map[i] : P.gain);
     const float offset = (float)(offset_map ? offset_map[i] : P.offset);
     float x = (float)k + noise;

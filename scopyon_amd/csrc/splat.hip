@@ -394,13 +394,14 @@ __device__ inline bool rec_tiles(const smx_fov_t& F, const EmitterRec& r, TileRa
 }
 
 __device__ inline int regular_phase(const Axis& a, double half, double xi, double ext_half,
-                                    double pl, double res, int ntab, int P, int M,
+                                    double pl, double res, double inv_res, int ntab, int P, int M,
                                     long long lo, long long hi, bool& regular) {
-    // phi = ceil(((lo + 1) pl - o) / res); check every boundary lo+1..hi
-    long long phi = smx::ceil_i64(((double)(lo + 1) * pl - a.o) / res);
+    // phi = ceil(((lo + 1) pl - o) / res); check every boundary lo+2..hi (the
+    // boundary lo+1 is phi itself, also under the clamp at hi)
+    long long phi = smx::ceil_div((double)(lo + 1) * pl - a.o, res, inv_res);
     if (phi < 1 || phi > P || hi - lo > M) regular = false;
-    for (long long k = lo + 1; regular && k <= hi; ++k) {
-        long long b = smx::ceil_i64(((double)k * pl - a.o) / res);
+    for (long long k = lo + 2; regular && k <= hi; ++k) {
+        long long b = smx::ceil_div((double)k * pl - a.o, res, inv_res);
         long long want = phi + (long long)P * (k - lo - 1);
         if (k == hi) {
             b = b > ntab ? ntab : b;
@@ -438,6 +439,7 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
     }
     const smx_fov_t F = fovs[f];
     const double xi = u[i] - F.p0u, yi = v[i] - F.p0v;
+    const double inv_res = 1.0 / res, inv_pl = 1.0 / F.pixel_length;
     bool regular = true;
     int ph[2];
     for (int ax = 0; ax < 2; ++ax) {
@@ -446,10 +448,10 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
         double base = half + x;
         Axis a;
         a.o = base - ext_half;
-        long long lo = (long long)floor(a.o / F.pixel_length);
-        long long hi = (long long)ceil((base + ext_half) / F.pixel_length);
-        ph[ax] = regular_phase(a, half, x, ext_half, F.pixel_length, res, ntab, P, M, lo, hi,
-                               regular);
+        long long lo = smx::floor_div(a.o, F.pixel_length, inv_pl);
+        long long hi = smx::ceil_div(base + ext_half, F.pixel_length, inv_pl);
+        ph[ax] = regular_phase(a, half, x, ext_half, F.pixel_length, res, inv_res, ntab, P, M, lo,
+                               hi, regular);
         if (lo < -(1LL << 30) || lo > (1LL << 30)) regular = false;
         long long span = hi - lo;
         short len = (short)(span < 0 ? 0 : (span > 32767 ? 32767 : span));
