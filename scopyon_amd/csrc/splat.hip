@@ -412,15 +412,13 @@ __device__ inline int regular_phase(const Axis& a, double half, double xi, doubl
     return (int)(phi - 1);
 }
 
-__global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const double* __restrict__ v,
-                                   const double* __restrict__ w, const int* __restrict__ key,
-                                   const int* __restrict__ fov_id, const smx_fov_t* __restrict__ fovs,
-                                   int nfov, double ext_half, double res, int ntab, int P, int M,
-                                   int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
-                                   EmitterRec* __restrict__ rec, int n_tiles,
-                                   int* __restrict__ tile_count, int cs, int* __restrict__ status) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Record of emitter i in FOV f (flags 2: no table slot or FOV, not binned).
+__device__ inline EmitterRec make_rec(int i, int f, const double* __restrict__ u,
+                                      const double* __restrict__ v, const double* __restrict__ w,
+                                      const int* __restrict__ key, const smx_fov_t* __restrict__ fovs,
+                                      int nfov, double ext_half, double res, int ntab, int P, int M,
+                                      int n_slots, size_t k_stride, int folded,
+                                      const smx::PhaseFold& PF, int* __restrict__ status) {
     EmitterRec r;
     r.lo_u = r.lo_v = 0;
     r.n_u = r.n_v = 0;
@@ -429,15 +427,13 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
     r.flags = 0;
     r.w = w[i];
     r.table_off = 0;
-    int f = fov_of(fov_id, i);
     int slot = key[i];
     if (!fov_ok(f, nfov) || slot < 0 || slot >= n_slots) {
         r.flags = 2;
-        rec[i] = r;
         if (fov_ok(f, nfov) && r.w > 0.0) atomicOr(status, 8);
-        return;
+        return r;
     }
-    const smx_fov_t F = fovs[f];
+    const smx_fov_t& F = fovs[f];
     const double xi = u[i] - F.p0u, yi = v[i] - F.p0v;
     const double inv_res = 1.0 / res, inv_pl = 1.0 / F.pixel_length;
     bool regular = true;
@@ -486,44 +482,120 @@ __global__ void emitter_rec_kernel(int n, const double* __restrict__ u, const do
         }
         r.table_off = (long long)slot * (long long)k_stride + off;
     }
-    rec[i] = r;
-    if (tile_count && r.w > 0.0) {
+    return r;
+}
+
+// Tile counts and slot claims aggregate in LDS over a window of REC_WIN tiles
+// from the first tile of the block's first FOV: one global atomic per block
+// and touched tile instead of one per emitter and tile (device-scope atomics
+// are the cost of binning).  Tiles outside the window go to global directly.
+constexpr int REC_THREADS = 1024;
+constexpr int REC_WIN = 4096;
+
+__device__ inline int window_base(const int* __restrict__ fov_id, const smx_fov_t* __restrict__ fovs,
+                                  int nfov, int n) {
+    const int i0 = blockIdx.x * REC_THREADS;
+    const int f0 = i0 < n ? fov_of(fov_id, i0) : -1;
+    return fov_ok(f0, nfov) ? fovs[f0].tile_off : 0;
+}
+
+__global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
+        int n, const double* __restrict__ u, const double* __restrict__ v,
+        const double* __restrict__ w, const int* __restrict__ key, const int* __restrict__ fov_id,
+        const smx_fov_t* __restrict__ fovs, int nfov, double ext_half, double res, int ntab, int P,
+        int M, int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
+        EmitterRec* __restrict__ rec, int n_tiles, int* __restrict__ tile_count, int cs,
+        int* __restrict__ status) {
+    __shared__ int s_hist[REC_WIN];
+    for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) s_hist[k] = 0;
+    const int wbase = window_base(fov_id, fovs, nfov, n);
+    __syncthreads();
+    const int i = blockIdx.x * REC_THREADS + threadIdx.x;
+    if (i < n) {
+        const int f = fov_of(fov_id, i);
+        const EmitterRec r = make_rec(i, f, u, v, w, key, fovs, nfov, ext_half, res, ntab, P, M,
+                                      n_slots, k_stride, folded, PF, status);
+        rec[i] = r;
         TileRange t;
-        if (rec_tiles(F, r, t))
+        if (tile_count && r.flags != 2 && r.w > 0.0 && rec_tiles(fovs[f], r, t)) {
+            const smx_fov_t& F = fovs[f];
             for (int a = t.r0; a <= t.r1; ++a)
                 for (int b = t.c0; b <= t.c1; ++b) {
                     const int tile = F.tile_off + a * F.tiles_c + b;
-                    if (tile >= 0 && tile < n_tiles) atomicAdd(&tile_count[(size_t)tile * cs], 1);
+                    if (tile < 0 || tile >= n_tiles) continue;
+                    const unsigned d = (unsigned)(tile - wbase);
+                    if (d < (unsigned)REC_WIN) atomicAdd(&s_hist[d], 1);
+                    else atomicAdd(&tile_count[(size_t)tile * cs], 1);
                 }
+        }
     }
+    __syncthreads();
+    if (tile_count)
+        for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) {
+            const int c = s_hist[k];
+            if (c) atomicAdd(&tile_count[(size_t)(wbase + k) * cs], c);
+        }
 }
 
 // Tile lists from the records (phase path): the footprint is the record's
 // pixel rectangle, already computed with the reference's float64 boundaries.
-__global__ void bin_fill_rec_kernel(int n, const EmitterRec* __restrict__ rec,
-                                    const int* __restrict__ fov_id,
-                                    const smx_fov_t* __restrict__ fovs, int nfov, int n_tiles,
-                                    const int* __restrict__ tile_begin,
-                                    int* __restrict__ tile_left, int cs, int* __restrict__ list,
-                                    long long capacity, int* __restrict__ status) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int f = fov_of(fov_id, i);
-    if (!fov_ok(f, nfov)) return;
-    const EmitterRec r = rec[i];
-    if (r.flags == 2 || !(r.w > 0.0)) return;
-    const smx_fov_t F = fovs[f];
-    TileRange t;
-    if (!rec_tiles(F, r, t)) return;
-    for (int a = t.r0; a <= t.r1; ++a)
-        for (int b = t.c0; b <= t.c1; ++b) {
-            const int tile = F.tile_off + a * F.tiles_c + b;
-            if (tile < 0 || tile >= n_tiles) continue;
-            const int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;   // counts run down to 0
-            const long long at = (long long)tile_begin[tile] + pos;
-            if (at < capacity) list[at] = i;
-            else atomicOr(status, 2);
+// Entries of a footprint within 2 x 2 tiles of the window take an LDS rank,
+// the block claims each window tile's slots with one global atomic, and the
+// entries are written at claim + rank; other entries claim a slot each.
+__global__ void __launch_bounds__(REC_THREADS) bin_fill_rec_kernel(
+        int n, const EmitterRec* __restrict__ rec, const int* __restrict__ fov_id,
+        const smx_fov_t* __restrict__ fovs, int nfov, int n_tiles,
+        const int* __restrict__ tile_begin, int* __restrict__ tile_left, int cs,
+        int* __restrict__ list, long long capacity, int* __restrict__ status) {
+    __shared__ int s_hist[REC_WIN];
+    for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) s_hist[k] = 0;
+    const int wbase = window_base(fov_id, fovs, nfov, n);
+    __syncthreads();
+    const int i = blockIdx.x * REC_THREADS + threadIdx.x;
+    int ent[4];   // window entries: tile offset << 16 | LDS rank (-1: none)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ent[q] = -1;
+    const int f = i < n ? fov_of(fov_id, i) : -1;
+    if (fov_ok(f, nfov)) {
+        const EmitterRec r = rec[i];
+        const smx_fov_t& F = fovs[f];
+        TileRange t;
+        if (r.flags != 2 && r.w > 0.0 && rec_tiles(F, r, t)) {
+            const bool small = t.r1 - t.r0 <= 1 && t.c1 - t.c0 <= 1;
+            int q = 0;
+            for (int a = t.r0; a <= t.r1; ++a)
+                for (int b = t.c0; b <= t.c1; ++b) {
+                    const int tile = F.tile_off + a * F.tiles_c + b;
+                    if (tile < 0 || tile >= n_tiles) continue;
+                    const unsigned d = (unsigned)(tile - wbase);
+                    if (small && d < (unsigned)REC_WIN) {
+                        // ranks stay below 2^16: at most REC_THREADS entries per tile
+                        ent[q & 3] = (int)(d << 16) | atomicAdd(&s_hist[d], 1);
+                        ++q;
+                    } else {
+                        const int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;
+                        const long long at = (long long)tile_begin[tile] + pos;
+                        if (at < capacity) list[at] = i;
+                        else atomicOr(status, 2);
+                    }
+                }
         }
+    }
+    __syncthreads();
+    // claim each window tile's slots (counts run down to 0): [old - c, old)
+    for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) {
+        const int c = s_hist[k];
+        if (c) s_hist[k] = atomicSub(&tile_left[(size_t)(wbase + k) * cs], c) - c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (ent[q] < 0) continue;
+        const int d = ent[q] >> 16, rank = ent[q] & 0xffff;
+        const long long at = (long long)tile_begin[wbase + d] + s_hist[d] + rank;
+        if (at < capacity) list[at] = i;
+        else atomicOr(status, 2);
+    }
 }
 
 // K[slot][phi_u][phi_v][du][dv] from the SATs (same arithmetic as the tile kernel).
@@ -916,6 +988,7 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     layout_workspace(n, n_tiles, list_capacity, (char*)workspace, &ws);
     const double ext_half = (res * (double)(ntab - 1)) * 0.5;   // signal_width * 0.5
     const int nb = (n + 255) / 256;
+    const int nbr = (n + REC_THREADS - 1) / REC_THREADS;
     // about two tiles per emitter at these footprints
     const int cs = smx::count_stride(2LL * n, n_tiles);
 
@@ -927,9 +1000,9 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
             if (K) {
                 // records first; they count their tiles as they are written
                 const int folded = k_format == 2;
-                hipLaunchKernelGGL(emitter_rec_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
-                                   key, fov_id, fovs_dev, nfov, ext_half, res, ntab, sat_period, m,
-                                   n_slots,
+                hipLaunchKernelGGL(emitter_rec_kernel, dim3(nbr), dim3(REC_THREADS), 0, stream, n,
+                                   u, v, w, key, fov_id, fovs_dev, nfov, ext_half, res, ntab,
+                                   sat_period, m, n_slots,
                                    folded ? PF.elems() : smx_phase_table_elems(sat_period, m),
                                    folded, PF, ws.rec, n_tiles, ws.tile_count, cs, status_dev);
                 SMX_CHECK_LAUNCH("emitter_rec_kernel");
@@ -949,8 +1022,8 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
             return SMX_ERUNTIME;
         if (n > 0) {
             if (K)
-                hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nb), dim3(256), 0, stream, n, ws.rec,
-                                   fov_id, fovs_dev, nfov, n_tiles, ws.tile_begin,
+                hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nbr), dim3(REC_THREADS), 0, stream, n,
+                                   ws.rec, fov_id, fovs_dev, nfov, n_tiles, ws.tile_begin,
                                    ws.tile_count, cs, ws.list, list_capacity, status_dev);
             else
                 hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
