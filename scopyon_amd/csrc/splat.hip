@@ -485,12 +485,11 @@ __device__ inline EmitterRec make_rec(int i, int f, const double* __restrict__ u
     return r;
 }
 
-// Tile counts and slot claims aggregate in LDS over a window of REC_WIN tiles
-// from the first tile of the block's first FOV: one global atomic per block
-// and touched tile instead of one per emitter and tile (device-scope atomics
-// are the cost of binning).  Tiles outside the window go to global directly.
-constexpr int REC_THREADS = 1024;
-constexpr int REC_WIN = 4096;
+// Tile counts and slot claims aggregate per block in LDS (BIN_THREADS
+// emitters, a window of BIN_WIN tiles from the first tile of the block's
+// first FOV): device-scope atomics are the cost of binning.
+constexpr int REC_THREADS = smx::BIN_THREADS;
+constexpr int REC_WIN = smx::BIN_WIN;
 
 __device__ inline int window_base(const int* __restrict__ fov_id, const smx_fov_t* __restrict__ fovs,
                                   int nfov, int n) {
@@ -507,7 +506,7 @@ __global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
         EmitterRec* __restrict__ rec, int n_tiles, int* __restrict__ tile_count, int cs,
         int* __restrict__ status) {
     __shared__ int s_hist[REC_WIN];
-    for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) s_hist[k] = 0;
+    smx::bin_hist_clear(s_hist);
     const int wbase = window_base(fov_id, fovs, nfov, n);
     __syncthreads();
     const int i = blockIdx.x * REC_THREADS + threadIdx.x;
@@ -522,19 +521,13 @@ __global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
             for (int a = t.r0; a <= t.r1; ++a)
                 for (int b = t.c0; b <= t.c1; ++b) {
                     const int tile = F.tile_off + a * F.tiles_c + b;
-                    if (tile < 0 || tile >= n_tiles) continue;
-                    const unsigned d = (unsigned)(tile - wbase);
-                    if (d < (unsigned)REC_WIN) atomicAdd(&s_hist[d], 1);
-                    else atomicAdd(&tile_count[(size_t)tile * cs], 1);
+                    if (tile >= 0 && tile < n_tiles)
+                        smx::bin_count(s_hist, wbase, tile, tile_count, cs);
                 }
         }
     }
     __syncthreads();
-    if (tile_count)
-        for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) {
-            const int c = s_hist[k];
-            if (c) atomicAdd(&tile_count[(size_t)(wbase + k) * cs], c);
-        }
+    if (tile_count) smx::bin_count_flush(s_hist, wbase, tile_count, cs);
 }
 
 // Tile lists from the records (phase path): the footprint is the record's
@@ -548,7 +541,7 @@ __global__ void __launch_bounds__(REC_THREADS) bin_fill_rec_kernel(
         const int* __restrict__ tile_begin, int* __restrict__ tile_left, int cs,
         int* __restrict__ list, long long capacity, int* __restrict__ status) {
     __shared__ int s_hist[REC_WIN];
-    for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) s_hist[k] = 0;
+    smx::bin_hist_clear(s_hist);
     const int wbase = window_base(fov_id, fovs, nfov, n);
     __syncthreads();
     const int i = blockIdx.x * REC_THREADS + threadIdx.x;
@@ -582,11 +575,7 @@ __global__ void __launch_bounds__(REC_THREADS) bin_fill_rec_kernel(
         }
     }
     __syncthreads();
-    // claim each window tile's slots (counts run down to 0): [old - c, old)
-    for (int k = threadIdx.x; k < REC_WIN; k += REC_THREADS) {
-        const int c = s_hist[k];
-        if (c) s_hist[k] = atomicSub(&tile_left[(size_t)(wbase + k) * cs], c) - c;
-    }
+    smx::bin_claim(s_hist, wbase, tile_left, cs);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {

@@ -165,6 +165,42 @@ __device__ inline void wave_sort_list(const int* src, int count, int lane, int* 
     else wave_sort<16>(src, count, lane, out);
 }
 
+// Binning with per-block aggregation: a block of BIN_THREADS emitters counts
+// (and later ranks) its tile entries in LDS over a window of BIN_WIN tiles
+// from `wbase`, then touches each window tile's global counter once, so the
+// device-scope atomics per tile drop from one per entry to one per block.
+// Entries outside the window use the global counter directly.
+constexpr int BIN_THREADS = 1024;
+constexpr int BIN_WIN = 4096;
+
+__device__ inline void bin_hist_clear(int* s_hist) {
+    for (int k = threadIdx.x; k < BIN_WIN; k += BIN_THREADS) s_hist[k] = 0;
+}
+// count one entry of `tile`
+__device__ inline void bin_count(int* s_hist, int wbase, int tile, int* __restrict__ tile_count,
+                                 int cs) {
+    const unsigned d = (unsigned)(tile - wbase);
+    if (d < (unsigned)BIN_WIN) atomicAdd(&s_hist[d], 1);
+    else atomicAdd(&tile_count[(size_t)tile * cs], 1);
+}
+// after a barrier: add the block's window counts to the global counters
+__device__ inline void bin_count_flush(const int* s_hist, int wbase, int* __restrict__ tile_count,
+                                       int cs) {
+    for (int k = threadIdx.x; k < BIN_WIN; k += BIN_THREADS) {
+        const int c = s_hist[k];
+        if (c) atomicAdd(&tile_count[(size_t)(wbase + k) * cs], c);
+    }
+}
+// after a barrier: claim each window tile's slots from the counters, which
+// run down to 0 -- the block's entries of tile wbase + k take [s_hist[k],
+// s_hist[k] + count) on return
+__device__ inline void bin_claim(int* s_hist, int wbase, int* __restrict__ tile_left, int cs) {
+    for (int k = threadIdx.x; k < BIN_WIN; k += BIN_THREADS) {
+        const int c = s_hist[k];
+        if (c) s_hist[k] = atomicSub(&tile_left[(size_t)(wbase + k) * cs], c) - c;
+    }
+}
+
 __device__ inline void lds_add(double* p, double v) {
     // ds_add_f64: one IEEE add in LDS, in program order per wave
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
