@@ -176,25 +176,49 @@ __device__ inline float2 normal2_f(uint32_t a, uint32_t b) {
     return make_float2(r * __builtin_amdgcn_cosf(t), r * __builtin_amdgcn_sinf(t));
 }
 
+// log(k!) in float32: the correctly rounded value for k < 10, Stirling's
+// series with two correction terms for k >= 10 (truncation error < 5e-9).
+__device__ inline float log_factorial(float kf) {
+    const float m = kf + 1.0f, r = __builtin_amdgcn_rcpf(m);
+    const float st = (m - 0.5f) * ln_hw(m) - m + 0x1.d67f1cp-1f +   // 0.5 log(2 pi)
+                     r * (1.0f / 12.0f - (r * r) * (1.0f / 360.0f));
+    const int k = (int)kf;
+    float t = 0.0f;
+    t = k == 2 ? 0x1.62e430p-1f : t;
+    t = k == 3 ? 0x1.cab0c0p+0f : t;
+    t = k == 4 ? 0x1.96ca78p+1f : t;
+    t = k == 5 ? 0x1.326644p+2f : t;
+    t = k == 6 ? 0x1.a51274p+2f : t;
+    t = k == 7 ? 0x1.10ce20p+3f : t;
+    t = k == 8 ? 0x1.5358e8p+3f : t;
+    t = k == 9 ? 0x1.99a892p+3f : t;
+    return kf >= 10.0f ? st : t;
+}
+
 // Poisson for lam >= 12: Hormann's PTRS (transformed rejection with squeeze,
-// Insurance: Mathematics and Economics 12, 1993).
+// Insurance: Mathematics and Economics 12, 1993), two proposals per Philox
+// block.  log(lam) stays the accurate logf (it is multiplied by k in the
+// acceptance test); the other logarithms run on v_log_f32.
 __device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
     const float slam = sqrtf(lam), loglam = logf(lam);
     const float b = 0.931f + 2.53f * slam;
     const float aa = -0.059f + 0.02483f * b;
-    const float invalpha = 1.1239f + 1.1328f / (b - 3.4f);
+    const float log_invalpha = ln_hw(1.1239f + 1.1328f / (b - 3.4f));
     const float vr = 0.9277f - 3.6224f / (b - 2.0f);
-    for (uint32_t it = 0; it < 512; ++it) {
-        smx::u32x4 r = R.block(1u + (it >> 1));
-        float U = u01f((it & 1) ? r.z : r.x) - 0.5f;
-        float V = u01f((it & 1) ? r.w : r.y);
-        float us = 0.5f - fabsf(U);
-        float kf = floorf((2.0f * aa / us + b) * U + lam + 0.43f);
-        if (us >= 0.07f && V <= vr) return (int)kf;
-        if (kf < 0.0f || (us < 0.013f && V > us)) continue;
-        if (logf(V) + logf(invalpha) - logf(aa / (us * us) + b) <=
-            -lam + kf * loglam - lgammaf(kf + 1.0f))
-            return (int)kf;
+    for (uint32_t blk = 1; blk <= 256; ++blk) {
+        const smx::u32x4 r = R.block(blk);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float U = u01f(h ? r.z : r.x) - 0.5f;
+            const float V = u01f(h ? r.w : r.y);
+            const float us = 0.5f - fabsf(U);
+            const float kf = floorf((2.0f * aa * __builtin_amdgcn_rcpf(us) + b) * U + lam + 0.43f);
+            if (us >= 0.07f && V <= vr) return (int)kf;
+            if (kf < 0.0f || (us < 0.013f && V > us)) continue;
+            if (ln_hw(V) + log_invalpha - ln_hw(aa / (us * us) + b) <=
+                -lam + kf * loglam - log_factorial(kf))
+                return (int)kf;
+        }
     }
     return (int)floorf(lam + 0.5f);
 }

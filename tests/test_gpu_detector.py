@@ -145,3 +145,43 @@ def test_emccd_philox_readout_noise_is_gaussian(gpu):
     assert abs(x.mean()) < 5 / numpy.sqrt(n)
     assert abs(x.var() - 1.0) < 5 * numpy.sqrt(2.0 / n)
     assert stats.kstest(x, 'norm').pvalue > 1e-3
+
+
+@pytest.mark.parametrize('lam', [0.5, 5.0, 11.9, 12.0, 30.0, 150.0, 2000.0])
+def test_philox_poisson_sampler_distribution(gpu, lam):
+    """The device Poisson sampler (inversion below 12, PTRS from 12 up, shared
+    by the EMCCD and CCD/CMOS paths) against the Poisson pmf: a CCD readout
+    with no noise, unit gain and no offset returns the count itself.  Pearson
+    chi-square over bins of expected count >= 20 (262,144 draws, fixed seed)
+    must not reject at 1e-4; mean and variance within 5 standard errors."""
+    from scipy import stats
+    from scopyon_amd import _lib, _device
+    det = _lib.DetectorParamsT()
+    det.qe, det.background, det.background_switch, det.emgain = 1.0, 0.0, 0, 1
+    det.readout_noise, det.fullwell, det.adc_max = 0.0, 1e300, 1e300
+    det.offset, det.gain = 0.0, 1.0
+    n = 512 * 512
+    photons = torch.full((n,), lam, dtype=torch.float64, device=gpu)
+    adc = torch.empty(n, dtype=torch.float64, device=gpu)
+    _lib.call('smx_photon_detector_philox', n, photons.data_ptr(), None, None, det, None, None, 0,
+              4321, 5, 0, None, None, adc.data_ptr(), 0, _device.stream())
+    k = adc.cpu().numpy()
+    assert (k == numpy.floor(k)).all() and (k >= 0).all()
+    assert abs(k.mean() - lam) < 5 * numpy.sqrt(lam / n)
+    assert abs(k.var() / lam - 1.0) < 5 * numpy.sqrt(2.0 / n) + 1e-3
+    # bins [lo, hi) merged from both tails until each expects >= 20 draws
+    ks = numpy.arange(0, int(lam + 12 * numpy.sqrt(lam) + 20))
+    pmf = stats.poisson.pmf(ks, lam)
+    edges, acc = [0], 0.0
+    for i, p in enumerate(pmf):
+        acc += p
+        if acc * n >= 20 and (1.0 - pmf[:i + 1].sum()) * n >= 20:
+            edges.append(i + 1)
+            acc = 0.0
+    edges.append(len(ks) + 10 ** 9)
+    obs = numpy.histogram(k, bins=numpy.array(edges, dtype=float) - 0.5)[0]
+    cdf = stats.poisson.cdf(numpy.array(edges[1:-1]) - 1, lam)
+    exp = numpy.diff(numpy.concatenate([[0.0], cdf, [1.0]])) * n
+    chi2 = float(((obs - exp) ** 2 / exp).sum())
+    pval = stats.chi2.sf(chi2, len(exp) - 1)
+    assert pval > 1e-4, (lam, chi2, len(exp), pval)
