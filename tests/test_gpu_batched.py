@@ -40,6 +40,28 @@ def test_uneven_fovs_match_oracle(gpu, precision, rtol):
             numpy.testing.assert_allclose(got, ref, rtol=rtol, atol=rtol * 1e-2 * ref.max())
 
 
+def test_many_fovs_span_the_binning_window(gpu):
+    """70 FOVs of 256^2 (64 tiles each) with 15 emitters each: the first block
+    of 1024 emitters spans more tiles than its LDS counting window (4096), so
+    tile counts and slot claims mix the LDS-aggregated and the direct global
+    path; every FOV still equals the oracle."""
+    from scopyon_amd.batched import BatchedSimulator
+    shape, nf, k = (256, 256), 70, 15
+    sim = BatchedSimulator(_cfg(shape), [shape] * nf, [k] * nf, dt=0.033, seed=5)
+    assert sim.n_tiles > 4096 and nf * k > 1024
+    sim.form()
+    assert sim.status() & ~1 == 0
+    u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
+    p = oracle_params({'detector': {'image_size': list(shape)}, 'magnification': 100.0})
+    cache = O.PSFCache(p)
+    for f in range(nf):
+        a, b = sim.emitter_offsets[f], sim.emitter_offsets[f + 1]
+        parts = numpy.column_stack([numpy.zeros(k), u[a:b], v[a:b], numpy.arange(k), numpy.ones(k)])
+        ref, _, _ = O.molecule_plane(p, cache, parts, shape, numpy.zeros(3), 0.033, {}, None, None)
+        got = sim.frame(f, 'photons').cpu().numpy()
+        numpy.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-8 * ref.max(), err_msg=str(f))
+
+
 def test_graph_replay_equals_eager(gpu):
     from scopyon_amd.batched import BatchedSimulator
     shapes = [(64, 64), (32, 96)]

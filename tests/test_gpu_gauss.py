@@ -57,6 +57,27 @@ def test_forward_and_nll_match_oracle(gpu):
     numpy.testing.assert_allclose(grad.cpu().numpy(), 1.0 - kd.cpu().numpy() / mu_h, rtol=1e-6)
 
 
+@pytest.mark.parametrize('method', ['lds', 'mfma-f32'])
+def test_many_frames_span_the_binning_window(gpu, method):
+    """72 frames of 256^2 (64 tiles each, 4608 in all) with 16 emitters each:
+    a 1024-emitter binning block spans more tiles than its LDS counting window
+    (4096), so tile counts and slot claims mix the LDS-aggregated and the
+    direct global path; every frame still equals the oracle."""
+    shapes = [(256, 256)] * 72
+    x, y, z, I, fid, _ = _case(5, shapes, 16)
+    m = _model(shapes)
+    assert m.n_tiles > 4096 and len(x) > 1024
+    zt = _dev(z) if method == 'lds' else None
+    mu = m.forward(_dev(x), _dev(y), zt, _dev(I), _dev(fid, torch.int32), method=method)
+    assert m.check_status() & ~1 == 0
+    zr = z if method == 'lds' else numpy.zeros_like(z)
+    ref = G.render(shapes, x, y, zr, I, fid, **KW)
+    for f in range(len(shapes)):
+        got = m.frame(mu, f).cpu().numpy().astype(float)
+        numpy.testing.assert_allclose(got, ref[f], rtol=2e-5, atol=1e-5 * ref[f].max(),
+                                      err_msg=str(f))
+
+
 def test_forward_is_deterministic(gpu):
     shapes = [(96, 96)]
     x, y, z, I, fid, _ = _case(1, shapes, 400)
