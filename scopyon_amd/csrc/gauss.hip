@@ -360,9 +360,9 @@ struct BwdArgs {
 };
 
 __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
+    __shared__ float4 s_rowf[16][WMAX];   // per group, window row k: (G, dG/dx, dG/dsigma)
     const int e = blockIdx.x * 16 + (threadIdx.x >> 4);
     const int j = threadIdx.x & 15;
-    const int gl = (threadIdx.x & 63) & ~15;    // first lane of this emitter's group
     const int R = A.P.radius, W = 2 * R + 1;
     const float sigma = A.P.sigma, inv_sigma = 1.0f / sigma, s = INV_SQRT2 * inv_sigma;
     bool valid = e < A.n;
@@ -395,18 +395,25 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
         sgxr = -(a1 * na1 - a0 * na0) * inv_sigma;      // dG/dsigma
         sgyc = -(b1 * nb1 - b0 * nb0) * inv_sigma;
     }
+    // the group's row factors go through LDS: one 16-byte broadcast read per
+    // row instead of three cross-lane shuffles
+    float4* rowf = s_rowf[threadIdx.x >> 4];
+    if (j < W) rowf[j] = make_float4(gxr, dgxr, sgxr, 0.0f);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int col = lv + j;
     const bool col_in = valid && j < W && col >= 0 && col < F.nh;
     float t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
     for (int k = 0; k < W; ++k) {
-        const float gk = __shfl(gxr, gl + k), dk = __shfl(dgxr, gl + k), sk = __shfl(sgxr, gl + k);
+        const float4 fk = rowf[k];
         const int row = lu + k;
         float g = 0.0f;
         if (col_in && row >= 0 && row < F.nw)
             g = A.grad_mu[(size_t)F.pix_off + (size_t)row * F.nh + col];
-        t1 += g * gk;
-        t2 += g * dk;
-        t3 += g * sk;
+        t1 += g * fk.x;
+        t2 += g * fk.y;
+        t3 += g * fk.z;
     }
     float pI = t1 * gyc, px = t2 * gyc, py = t1 * dgyc, ps = t3 * gyc + t1 * sgyc;
 #pragma unroll
