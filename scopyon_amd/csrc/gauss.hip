@@ -55,6 +55,15 @@ __device__ inline float g_pixel(float i, float x, float s) {
     return 0.5f * (erff(t1) - erff(t0));
 }
 
+// g_pixel from the pixel's edge arguments t0 < t1 and e0 = erfc(|t0|),
+// e1 = erfc(|t1|): the same value as g_pixel (the straddling pixel takes the
+// erf branch itself).
+__device__ inline float g_pixel_edges(float t0, float t1, float e0, float e1) {
+    if (t0 >= 0.0f) return 0.5f * (e0 - e1);
+    if (t1 <= 0.0f) return 0.5f * (e1 - e0);
+    return 0.5f * (erff(t1) - erff(t0));
+}
+
 // Normal density N(t) = exp(-t^2 / (2 sigma^2)) / (sqrt(2 pi) sigma).
 __device__ inline float normal_pdf(float t, float inv_sigma) {
     float u = t * inv_sigma;
@@ -381,19 +390,31 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
         valid = finite_coord(xe) && finite_coord(ye);
     }
     const int lu = valid ? (int)floorf(xe) - R : 0, lv = valid ? (int)floorf(ye) - R : 0;
+    // window edge j (lane j, j = 0..W) along both axes: offset from the
+    // emitter, erfc(|t|) and the normal density; pixel j combines edges j and
+    // j + 1 (the next lane of the group), so every lane evaluates one erfc and
+    // one exp per axis instead of two erf-type pairs per pixel
+    float ax = 0.0f, bx = 0.0f, eax = 0.0f, ebx = 0.0f, nax = 0.0f, nbx = 0.0f;
+    if (valid && j <= W) {
+        ax = (float)(lu + j) - xe;
+        bx = (float)(lv + j) - ye;
+        eax = erfcf(fabsf(ax * s));
+        ebx = erfcf(fabsf(bx * s));
+        nax = normal_pdf(ax, inv_sigma);
+        nbx = normal_pdf(bx, inv_sigma);
+    }
+    const float ax1 = __shfl_down(ax, 1, 16), bx1 = __shfl_down(bx, 1, 16);
+    const float eax1 = __shfl_down(eax, 1, 16), ebx1 = __shfl_down(ebx, 1, 16);
+    const float nax1 = __shfl_down(nax, 1, 16), nbx1 = __shfl_down(nbx, 1, 16);
     // this lane's window row factors (row lu + j) and column factors (col lv + j)
     float gxr = 0.0f, dgxr = 0.0f, sgxr = 0.0f, gyc = 0.0f, dgyc = 0.0f, sgyc = 0.0f;
     if (valid && j < W) {
-        const float i0 = (float)(lu + j), j0 = (float)(lv + j);
-        gxr = g_pixel(i0, xe, s);
-        gyc = g_pixel(j0, ye, s);
-        const float a0 = i0 - xe, a1 = a0 + 1.0f, b0 = j0 - ye, b1 = b0 + 1.0f;
-        const float na0 = normal_pdf(a0, inv_sigma), na1 = normal_pdf(a1, inv_sigma);
-        const float nb0 = normal_pdf(b0, inv_sigma), nb1 = normal_pdf(b1, inv_sigma);
-        dgxr = na0 - na1;                               // dG/dx
-        dgyc = nb0 - nb1;
-        sgxr = -(a1 * na1 - a0 * na0) * inv_sigma;      // dG/dsigma
-        sgyc = -(b1 * nb1 - b0 * nb0) * inv_sigma;
+        gxr = g_pixel_edges(ax * s, ax1 * s, eax, eax1);
+        gyc = g_pixel_edges(bx * s, bx1 * s, ebx, ebx1);
+        dgxr = nax - nax1;                                  // dG/dx
+        dgyc = nbx - nbx1;
+        sgxr = -(ax1 * nax1 - ax * nax) * inv_sigma;        // dG/dsigma
+        sgyc = -(bx1 * nbx1 - bx * nbx) * inv_sigma;
     }
     // the group's row factors go through LDS: one 16-byte broadcast read per
     // row instead of three cross-lane shuffles
