@@ -398,8 +398,18 @@ __device__ inline int regular_phase(const Axis& a, double half, double xi, doubl
                                     long long lo, long long hi, bool& regular) {
     // phi = ceil(((lo + 1) pl - o) / res); check every boundary lo+2..hi (the
     // boundary lo+1 is phi itself, also under the clamp at hi)
-    long long phi = smx::ceil_div((double)(lo + 1) * pl - a.o, res, inv_res);
+    const double q1 = (double)(lo + 1) * pl - a.o;
+    long long phi = smx::ceil_div(q1, res, inv_res);
     if (phi < 1 || phi > P || hi - lo > M) regular = false;
+    // Shortcut, exact: with pl / res = P to 1e-13, boundary k's quotient is
+    // q1 / res + P (k - lo - 1) to within 3e-10 (k - lo <= M <= 32, quotients
+    // below 1e6), so a first quotient at least 1e-8 from an integer fixes
+    // every ceiling to phi + P (k - lo - 1): the loop below would find all of
+    // them regular.  Otherwise each boundary is checked.
+    const double qq = q1 * inv_res, fr = qq - floor(qq);
+    if (regular && M <= 32 && fabs(qq) < 1e6 && fr > 1e-8 && fr < 1.0 - 1e-8 &&
+        fabs(pl * inv_res - (double)P) < 1e-13 * (double)P)
+        return (int)(phi - 1);
     for (long long k = lo + 2; regular && k <= hi; ++k) {
         long long b = smx::ceil_div((double)k * pl - a.o, res, inv_res);
         long long want = phi + (long long)P * (k - lo - 1);
