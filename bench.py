@@ -326,7 +326,9 @@ def batched_uneven_leg(steps=20, warmup=3):
             'fovs': F, 'emitters': int(counts.sum()), 'pixels': int(sim.n_pix),
             'frames_per_sec': F / sec, 'ms_per_step': sec * 1e3,
             'emitter_frames_per_sec': float(counts.sum()) / sec,
-            'status_unsorted_lists': bool(sim.status() & 1)}
+            'status_unsorted_lists': bool(sim.status() & 1),
+            'tile_side': sim.tile_side,
+            'tile_autotune_ms': {str(k): v for k, v in sim.tile_timings.items()}}
 
 
 def cpu_baseline(size, n_emit, sample_emitters, seed=1):
@@ -472,7 +474,9 @@ def main():
                    'psf': 'TRITC Born-Wolf; folded float32 phase table from the float64 SAT; '
                           'float64 accumulation',
                    'dt_s': dt, 'D_um2_s': 0.1, 'walls': 'reflecting',
-                   'emccd': 'Philox4x32-10, float32 sampler'},
+                   'emccd': 'Philox4x32-10, float32 sampler',
+                   'tile_side': sim.tile_side,
+                   'tile_autotune_ms': {str(k): v for k, v in sim.tile_timings.items()}},
         # plain = the same kernels, one frame per step, eager launches
         'plain_frames_per_sec': plain_fps,
         'batched_over_plain': batched_fps / plain_fps,

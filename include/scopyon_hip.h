@@ -138,6 +138,33 @@ int smx_splat_accumulate(int n, const double* u, const double* v, const double* 
                          int accumulate, const void* K, int k_format, int m, double* out,
                          void* workspace, size_t workspace_bytes, int* status_dev,
                          hipStream_t stream);
+/* The same three with the tile side as the last argument (16, 32 or 64 with
+ * phase tables K, 32 without): the FOV descriptors' tiles_r / tiles_c and
+ * n_tiles must be counted in tiles of that side.  The plain entry points use
+ * 32 (smx_splat_tile_size()).  A tile of side 16 q is processed by q^2 waves,
+ * one per 16 x 16 quadrant; the batched path times the three on its own
+ * workload and keeps the fastest.  No reference counterpart (the reference
+ * loops over emitters on the whole frame, _epifm.py:215-282). */
+int smx_splat_expected_ts(int n, const double* u, const double* v, const double* w,
+                          const int* key, const int* fov_id, const smx_fov_t* fovs_dev, int nfov,
+                          int n_tiles, const double* sat, size_t sat_stride, int n_slots,
+                          int ntab, int sat_period, double res, double unit_area,
+                          long long list_capacity, int accumulate, const void* K, int k_format,
+                          int m, double* out, void* workspace, size_t workspace_bytes,
+                          int* status_dev, hipStream_t stream, int tile_side);
+int smx_splat_bin_ts(int n, const double* u, const double* v, const double* w, const int* key,
+                     const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
+                     const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
+                     double res, double unit_area, long long list_capacity, int accumulate,
+                     const void* K, int k_format, int m, double* out, void* workspace,
+                     size_t workspace_bytes, int* status_dev, hipStream_t stream, int tile_side);
+int smx_splat_accumulate_ts(int n, const double* u, const double* v, const double* w,
+                            const int* key, const int* fov_id, const smx_fov_t* fovs_dev,
+                            int nfov, int n_tiles, const double* sat, size_t sat_stride,
+                            int n_slots, int ntab, int sat_period, double res, double unit_area,
+                            long long list_capacity, int accumulate, const void* K, int k_format,
+                            int m, double* out, void* workspace, size_t workspace_bytes,
+                            int* status_dev, hipStream_t stream, int tile_side);
 
 /* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */
 /* E = QE * (photons + background) (_epifm.py:1436-1441); flag_out (may be

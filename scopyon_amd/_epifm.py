@@ -234,9 +234,9 @@ class EPIFMConfigs:
 class FrameGeometry:
     """One camera frame as the C-ABI's smx_fov_t (plus derived splat sizes)."""
 
-    def __init__(self, nw, nh, pixel_length, p0u, p0v, pix_off=0, tile_off=0):
+    def __init__(self, nw, nh, pixel_length, p0u, p0v, pix_off=0, tile_off=0, tile=None):
         lib = _lib.lib()
-        ts = lib.smx_splat_tile_size()
+        ts = int(tile) if tile else lib.smx_splat_tile_size()
         f = _lib.FovT()
         f.nw, f.nh = int(nw), int(nh)
         f.tiles_r, f.tiles_c = (int(nw) + ts - 1) // ts, (int(nh) + ts - 1) // ts

@@ -115,6 +115,8 @@ SIGNATURES.update({
 })
 SIGNATURES['smx_splat_bin'] = SIGNATURES['smx_splat_expected']
 SIGNATURES['smx_splat_accumulate'] = SIGNATURES['smx_splat_expected']
+for _name in ('smx_splat_expected', 'smx_splat_bin', 'smx_splat_accumulate'):
+    SIGNATURES[_name + '_ts'] = (c_int, SIGNATURES[_name][1] + [c_int])
 
 
 def lib():

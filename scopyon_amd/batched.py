@@ -30,7 +30,8 @@ class BatchedSimulator:
 
     def __init__(self, config, fov_shapes, n_emitters, *, D=0.1e-12, dt=None, seed=0,
                  boundary='reflect', box=None, depth_range=None, method='default',
-                 adc_dtype=torch.float32, keep_expectation=False, table_precision='folded'):
+                 adc_dtype=torch.float32, keep_expectation=False, table_precision='folded',
+                 tile_side='auto'):
         self.device = _device.require_gpu()
         conf = config[method] if hasattr(config, '__getitem__') else config
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
@@ -49,18 +50,10 @@ class BatchedSimulator:
         self.n = int(self.emitter_offsets[-1])
         pl = c.pixel_length
         p0 = c.detector_focal_point
-        geoms, pix, tiles = [], 0, 0
-        for (nw, nh) in fov_shapes:
-            g = FrameGeometry(nw, nh, pl, p0[1], p0[2], pix_off=pix, tile_off=tiles)
-            geoms.append(g)
-            pix += int(nw) * int(nh)
-            tiles += g.n_tiles
-        self.geoms = geoms
-        self.n_pix, self.n_tiles = pix, tiles
-        self.pix_offsets = numpy.array([g.fov.pix_off for g in geoms] + [pix])
-        raw = b''.join(bytes(g.fov) for g in geoms)
-        self.fovs_dev = torch.from_numpy(numpy.frombuffer(raw, dtype=numpy.uint8).copy()).to(
-            self.device)
+        self.fov_shapes = [(int(nw), int(nh)) for nw, nh in fov_shapes]
+        self.tile_side = 32
+        self._set_geometry(32)
+        geoms = self.geoms
         # PSF tables: depth 0 only for 2D batches, every key for 3D batches
         self.tables = PSFTables(c.fluorophore_type, c.psf_wavelength, c.radial_cutoff,
                                 c.depth_cutoff, c.psf_radial_width, pl, self.device)
@@ -134,10 +127,9 @@ class BatchedSimulator:
         self.w = torch.empty(self.n, dtype=torch.float64, device=self.device)
         self.key = torch.empty(self.n, dtype=torch.int32, device=self.device)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
-        cap = max(1, self.n * tiles_per_emitter(pl, self.tables.ntab))
-        self.capacity = cap
-        self.ws = torch.empty(_lib.lib().smx_splat_workspace_bytes(self.n, self.n_tiles, cap),
-                              dtype=torch.uint8, device=self.device)
+        # tiles of side >= 16: at most tiles_per_emitter(..., 16) per emitter
+        self.capacity = max(1, self.n * tiles_per_emitter(pl, self.tables.ntab))
+        self._set_workspace()
         self.use_phase = self.tables.use_phase
         # 'float64': the phase tables themselves (same values as the SAT path);
         # 'float32': their float32 copy (half the table traffic); 'folded': the
@@ -156,6 +148,71 @@ class BatchedSimulator:
             else:
                 self.Ktab = self.tables.kfold()
         self.graph = None
+        self.tile_timings = {}
+        if tile_side == 'auto':
+            if self.use_phase:
+                self.autotune_tiles()
+        elif int(tile_side) != 32:
+            if int(tile_side) not in self.TILE_SIDES or not self.use_phase:
+                raise ValueError('tile_side must be 16, 32 or 64 (32 without phase tables)')
+            self._set_geometry(int(tile_side))
+            self._set_workspace()
+
+    # ---- tiles ----
+    TILE_SIDES = (16, 32, 64)
+
+    def _set_geometry(self, ts):
+        """FOV descriptors counted in tiles of side ts."""
+        pl, p0 = self.configs.pixel_length, self.configs.detector_focal_point
+        geoms, pix, tiles = [], 0, 0
+        for (nw, nh) in self.fov_shapes:
+            g = FrameGeometry(nw, nh, pl, p0[1], p0[2], pix_off=pix, tile_off=tiles, tile=ts)
+            geoms.append(g)
+            pix += nw * nh
+            tiles += g.n_tiles
+        self.geoms = geoms
+        self.n_pix, self.n_tiles = pix, tiles
+        self.pix_offsets = numpy.array([g.fov.pix_off for g in geoms] + [pix])
+        raw = b''.join(bytes(g.fov) for g in geoms)
+        self.fovs_dev = torch.from_numpy(numpy.frombuffer(raw, dtype=numpy.uint8).copy()).to(
+            self.device)
+        self.tile_side = ts
+
+    def _set_workspace(self):
+        self.ws = torch.empty(
+            _lib.lib().smx_splat_workspace_bytes(self.n, self.n_tiles, self.capacity),
+            dtype=torch.uint8, device=self.device)
+
+    def autotune_tiles(self, reps=3):
+        """Pick the splat tile side (16, 32 or 64 pixels: 1, 4 or 16 waves per
+        tile) by timing the binning + tile pass of this batch's current
+        positions with HIP events; a side whose tile lists overflow the sort
+        capacity (status bit 1) is taken only if every side does.  Returns
+        the side kept; the timings (ms) stay in `tile_timings`."""
+        if self.graph is not None:
+            raise RuntimeError('autotune_tiles() before capture()')
+        self.weights()
+        results = {}
+        for ts in self.TILE_SIDES:
+            self._set_geometry(ts)
+            self._set_workspace()
+            self.splatter.status.zero_()
+            self.splat()                           # warm (code objects, caches)
+            torch.cuda.synchronize()
+            unsorted = bool(int(self.splatter.status.item()) & 1)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                self.splat()
+            b.record()
+            torch.cuda.synchronize()
+            results[ts] = (unsorted, a.elapsed_time(b) / reps)
+        self.splatter.status.zero_()
+        self.tile_timings = {ts: ms for ts, (_, ms) in results.items()}
+        best = min(results, key=lambda ts: results[ts])   # sorted lists first, then time
+        self._set_geometry(best)
+        self._set_workspace()
+        return best
 
     # ---- one step ----
     def move(self):
@@ -180,15 +237,16 @@ class BatchedSimulator:
 
     def splat(self, entry='smx_splat_expected'):
         """Pixel integration of every FOV (entry: smx_splat_expected, or its two
-        halves smx_splat_bin / smx_splat_accumulate)."""
+        halves smx_splat_bin / smx_splat_accumulate), in tiles of side
+        `tile_side`."""
         t = self.tables
-        _lib.call(entry, self.n, self.u.data_ptr(), self.v.data_ptr(),
+        _lib.call(entry + '_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
                   self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.sat.data_ptr(),
                   t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 0,
                   _device.ptr(self.Ktab), self.k_format, t.m, self.photons.data_ptr(),
                   self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(),
-                  _device.stream())
+                  _device.stream(), self.tile_side)
 
     def detect(self):
         """EMCCD + ADC of every frame, then advance the frame counter."""

@@ -383,13 +383,15 @@ struct EmitterRec {
     long long table_off;     // element of pixel (du, dv) = table_off + du * su + dv * sv
 };
 
-// Tiles touched by a record's footprint rectangle [lo, lo + n) clipped to the frame.
-__device__ inline bool rec_tiles(const smx_fov_t& F, const EmitterRec& r, TileRange& t) {
+// Tiles (side 2^lts) touched by a record's footprint rectangle [lo, lo + n)
+// clipped to the frame.
+__device__ inline bool rec_tiles(const smx_fov_t& F, const EmitterRec& r, TileRange& t,
+                                 int lts = 5) {
     const int r0 = max(r.lo_u, 0), c0 = max(r.lo_v, 0);
     const int r1 = (int)min((long long)r.lo_u + r.n_u, (long long)F.nw) - 1;
     const int c1 = (int)min((long long)r.lo_v + r.n_v, (long long)F.nh) - 1;
     if (r1 < r0 || c1 < c0) return false;
-    t.r0 = r0 / TS; t.r1 = r1 / TS; t.c0 = c0 / TS; t.c1 = c1 / TS;
+    t.r0 = r0 >> lts; t.r1 = r1 >> lts; t.c0 = c0 >> lts; t.c1 = c1 >> lts;
     return true;
 }
 
@@ -514,7 +516,7 @@ __global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
         const smx_fov_t* __restrict__ fovs, int nfov, double ext_half, double res, int ntab, int P,
         int M, int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
         EmitterRec* __restrict__ rec, int n_tiles, int* __restrict__ tile_count, int cs,
-        int* __restrict__ status) {
+        int* __restrict__ status, int lts) {
     __shared__ int s_hist[REC_WIN];
     smx::bin_hist_clear(s_hist);
     const int wbase = window_base(fov_id, fovs, nfov, n);
@@ -526,7 +528,7 @@ __global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
                                       n_slots, k_stride, folded, PF, status);
         rec[i] = r;
         TileRange t;
-        if (tile_count && r.flags != 2 && r.w > 0.0 && rec_tiles(fovs[f], r, t)) {
+        if (tile_count && r.flags != 2 && r.w > 0.0 && rec_tiles(fovs[f], r, t, lts)) {
             const smx_fov_t& F = fovs[f];
             for (int a = t.r0; a <= t.r1; ++a)
                 for (int b = t.c0; b <= t.c1; ++b) {
@@ -549,7 +551,7 @@ __global__ void __launch_bounds__(REC_THREADS) bin_fill_rec_kernel(
         int n, const EmitterRec* __restrict__ rec, const int* __restrict__ fov_id,
         const smx_fov_t* __restrict__ fovs, int nfov, int n_tiles,
         const int* __restrict__ tile_begin, int* __restrict__ tile_left, int cs,
-        int* __restrict__ list, long long capacity, int* __restrict__ status) {
+        int* __restrict__ list, long long capacity, int* __restrict__ status, int lts) {
     __shared__ int s_hist[REC_WIN];
     smx::bin_hist_clear(s_hist);
     const int wbase = window_base(fov_id, fovs, nfov, n);
@@ -563,7 +565,7 @@ __global__ void __launch_bounds__(REC_THREADS) bin_fill_rec_kernel(
         const EmitterRec r = rec[i];
         const smx_fov_t& F = fovs[f];
         TileRange t;
-        if (r.flags != 2 && r.w > 0.0 && rec_tiles(F, r, t)) {
+        if (r.flags != 2 && r.w > 0.0 && rec_tiles(F, r, t, lts)) {
             const bool small = t.r1 - t.r0 <= 1 && t.c1 - t.c0 <= 1;
             int q = 0;
             for (int a = t.r0; a <= t.r1; ++a)
@@ -692,7 +694,8 @@ __device__ __forceinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& 
     return acc;
 }
 
-// One wave per 16 x 16 quadrant of the 32 x 32 tile.  Per chunk of the sorted
+// One wave per 16 x 16 quadrant of a tile of QPW x QPW quadrants (side 16,
+// 32 or 64: the tile shape the batched path's autotuner picks).  Per chunk of the sorted
 // tile list, each wave lists the 64-pixel parts of the emitters' footprints
 // that fall in its quadrant (lane = pixel of the footprint rectangle, so no
 // lane idles on pixels outside it), issues the table loads of NSLOT parts at
@@ -703,20 +706,22 @@ __device__ __forceinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& 
 // rounded to float32, relative error <= 2^-24).
 constexpr int QS = 16;            // quadrant side
 constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
-constexpr int PSORT_CAP = 2048;   // tile list sorted in LDS
+// tile list sorted in LDS: 2048 entries, 1024 for one-wave (16 x 16) tiles
+template <int QPW> constexpr int psort_cap() { return QPW == 1 ? 1024 : 2048; }
 
-template <typename KT>
-__global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
+template <typename KT, int QPW>
+__global__ void __launch_bounds__(64 * QPW * QPW) splat_phase_kernel(FastArgs A) {
+    constexpr int WPT = QPW * QPW, TSZ = QS * QPW, PSORT_CAP = psort_cap<QPW>();
     __shared__ int s_sort[PSORT_CAP];
-    __shared__ double s_acc[4][QS * QS];
+    __shared__ double s_acc[WPT][QS * QS];
     __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (frame pixels), n_u, n_v
     __shared__ int2 s_str[FCHUNK];            // table strides per pixel along u, v
     __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
     __shared__ double s_w[FCHUNK];
     __shared__ const KT* s_kb[FCHUNK];        // table element of footprint pixel (0, 0)
-    __shared__ int4 s_qi[4][FCHUNK];          // per wave: footprint rectangle in the quadrant
-    __shared__ const KT* s_qk[4][FCHUNK];     // per wave: its first table element
-    __shared__ int s_part[4][FCHUNK * 4];     // per wave: emitter << 2 | part
+    __shared__ int4 s_qi[WPT][FCHUNK];        // per wave: footprint rectangle in the quadrant
+    __shared__ const KT* s_qk[WPT][FCHUNK];   // per wave: its first table element
+    __shared__ int s_part[WPT][FCHUNK * 4];   // per wave: emitter << 2 | part
 
     const int tile = blockIdx.x;
     const int fov = A.tile_fov[tile];
@@ -725,7 +730,7 @@ __global__ void __launch_bounds__(256) splat_phase_kernel(FastArgs A) {
     const int local = tile - F.tile_off;
     const int tr = local / F.tiles_c, tc = local % F.tiles_c;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int qr0 = tr * TS + (wave >> 1) * QS, qc0 = tc * TS + (wave & 1) * QS;
+    const int qr0 = tr * TSZ + (wave / QPW) * QS, qc0 = tc * TSZ + (wave % QPW) * QS;
     const int qh = min(QS, F.nw - qr0), qw = min(QS, F.nh - qc0);
     const bool active = qh > 0 && qw > 0;
     double* acc = s_acc[wave];
@@ -967,8 +972,15 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                      const double* sat, size_t sat_stride, int n_slots, int ntab, int sat_period,
                      double res, double unit_area, long long list_capacity, int accumulate,
                      const void* K, int k_format, int m, double* out, void* workspace,
-                     size_t workspace_bytes, int* status_dev, hipStream_t stream, int stages) {
+                     size_t workspace_bytes, int* status_dev, hipStream_t stream, int stages,
+                     int tile_side) {
     const smx::PhaseFold PF = smx::make_phase_fold(ntab, sat_period > 0 ? sat_period : 1);
+    const int lts = tile_side == 16 ? 4 : tile_side == 32 ? 5 : tile_side == 64 ? 6 : -1;
+    if (lts < 0 || (!K && lts != 5)) {
+        smx::set_error("smx_splat_expected: tile side %d (16, 32 or 64 with phase tables; "
+                       "32 without)", tile_side);
+        return SMX_EVALUE;
+    }
     if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev || !out ||
         (K && (sat_period < 2 || m < 1 || k_format < 0 || k_format > 2)) ||
         (K && k_format == 2 && PF.mf > m) ||
@@ -1003,7 +1015,8 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                                    u, v, w, key, fov_id, fovs_dev, nfov, ext_half, res, ntab,
                                    sat_period, m, n_slots,
                                    folded ? PF.elems() : smx_phase_table_elems(sat_period, m),
-                                   folded, PF, ws.rec, n_tiles, ws.tile_count, cs, status_dev);
+                                   folded, PF, ws.rec, n_tiles, ws.tile_count, cs, status_dev,
+                                   lts);
                 SMX_CHECK_LAUNCH("emitter_rec_kernel");
             } else {
                 hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
@@ -1023,7 +1036,7 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
             if (K)
                 hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nbr), dim3(REC_THREADS), 0, stream, n,
                                    ws.rec, fov_id, fovs_dev, nfov, n_tiles, ws.tile_begin,
-                                   ws.tile_count, cs, ws.list, list_capacity, status_dev);
+                                   ws.tile_count, cs, ws.list, list_capacity, status_dev, lts);
             else
                 hipLaunchKernelGGL(bin_fill_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
                                    fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin,
@@ -1041,10 +1054,16 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         B.sat = sat; B.sat_stride = sat_stride; B.L = smx::make_sat_layout(ntab + 1, sat_period);
         B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
         B.accumulate = accumulate; B.out = out; B.status = status_dev;
-        if (k_format != 0)
-            hipLaunchKernelGGL(splat_phase_kernel<float>, dim3(n_tiles), dim3(256), 0, stream, B);
-        else
-            hipLaunchKernelGGL(splat_phase_kernel<double>, dim3(n_tiles), dim3(256), 0, stream, B);
+        const dim3 grid((unsigned)n_tiles);
+        if (k_format != 0) {
+            if (lts == 4) hipLaunchKernelGGL((splat_phase_kernel<float, 1>), grid, dim3(64), 0, stream, B);
+            else if (lts == 5) hipLaunchKernelGGL((splat_phase_kernel<float, 2>), grid, dim3(256), 0, stream, B);
+            else hipLaunchKernelGGL((splat_phase_kernel<float, 4>), grid, dim3(1024), 0, stream, B);
+        } else {
+            if (lts == 4) hipLaunchKernelGGL((splat_phase_kernel<double, 1>), grid, dim3(64), 0, stream, B);
+            else if (lts == 5) hipLaunchKernelGGL((splat_phase_kernel<double, 2>), grid, dim3(256), 0, stream, B);
+            else hipLaunchKernelGGL((splat_phase_kernel<double, 4>), grid, dim3(1024), 0, stream, B);
+        }
         SMX_CHECK_LAUNCH("splat_phase_kernel");
         return SMX_OK;
     }
@@ -1071,9 +1090,18 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         sat_period, res, unit_area, list_capacity, accumulate, K, k_format, m, out, workspace, \
         workspace_bytes, status_dev, stream
 
-int smx_splat_expected(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 3); }
-int smx_splat_bin(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 1); }
-int smx_splat_accumulate(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 2); }
+int smx_splat_expected(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 3, 32); }
+int smx_splat_bin(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 1, 32); }
+int smx_splat_accumulate(SMX_SPLAT_PARAMS) { return splat_run(SMX_SPLAT_ARGS, 2, 32); }
+int smx_splat_expected_ts(SMX_SPLAT_PARAMS, int tile_side) {
+    return splat_run(SMX_SPLAT_ARGS, 3, tile_side);
+}
+int smx_splat_bin_ts(SMX_SPLAT_PARAMS, int tile_side) {
+    return splat_run(SMX_SPLAT_ARGS, 1, tile_side);
+}
+int smx_splat_accumulate_ts(SMX_SPLAT_PARAMS, int tile_side) {
+    return splat_run(SMX_SPLAT_ARGS, 2, tile_side);
+}
 
 __global__ void to_float_kernel(const double* __restrict__ a, size_t n, float* __restrict__ b) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -47,7 +47,7 @@ def test_many_fovs_span_the_binning_window(gpu):
     path; every FOV still equals the oracle."""
     from scopyon_amd.batched import BatchedSimulator
     shape, nf, k = (256, 256), 70, 15
-    sim = BatchedSimulator(_cfg(shape), [shape] * nf, [k] * nf, dt=0.033, seed=5)
+    sim = BatchedSimulator(_cfg(shape), [shape] * nf, [k] * nf, dt=0.033, seed=5, tile_side=32)
     assert sim.n_tiles > 4096 and nf * k > 1024
     sim.form()
     assert sim.status() & ~1 == 0
@@ -60,6 +60,34 @@ def test_many_fovs_span_the_binning_window(gpu):
         ref, _, _ = O.molecule_plane(p, cache, parts, shape, numpy.zeros(3), 0.033, {}, None, None)
         got = sim.frame(f, 'photons').cpu().numpy()
         numpy.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-8 * ref.max(), err_msg=str(f))
+
+
+@pytest.mark.parametrize('precision', ['float64', 'folded'])
+def test_tile_sides_give_identical_frames(gpu, precision):
+    """Tiles of 16, 32 and 64 pixels (1, 4 and 16 waves per tile) sum every
+    pixel's emitters in emitter order, so the frames are bit-identical; the
+    autotuner times all three and keeps one."""
+    from scopyon_amd.batched import BatchedSimulator
+    shapes = [(64, 64), (48, 80), (128, 128), (16, 16), (200, 72)]
+    counts = [50, 10, 400, 3, 120]
+    frames = {}
+    for ts in (16, 32, 64):
+        sim = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, table_precision=precision,
+                               tile_side=ts)
+        assert sim.tile_side == ts
+        sim.form()
+        assert sim.status() & ~1 == 0
+        frames[ts] = sim.photons.cpu().numpy()
+    assert frames[32].any()
+    for ts in (16, 64):
+        assert numpy.array_equal(frames[ts].view(numpy.uint64), frames[32].view(numpy.uint64)), ts
+    auto = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, table_precision=precision)
+    assert auto.tile_side in (16, 32, 64) and sorted(auto.tile_timings) == [16, 32, 64]
+    auto.form()
+    assert numpy.array_equal(auto.photons.cpu().numpy().view(numpy.uint64),
+                             frames[32].view(numpy.uint64))
+    with pytest.raises(ValueError):
+        BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, tile_side=24)
 
 
 def test_graph_replay_equals_eager(gpu):
