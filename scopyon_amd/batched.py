@@ -13,6 +13,12 @@ Layout in HBM (structure of arrays): emitter coordinates ``depth, u, v``
 (int32 per emitter), per-emitter ``D``; frames concatenated at
 ``pix_off`` offsets: photons/expectation (float64) and ADC counts (float32 by
 default).  The PSF summed-area tables are shared by all FOVs.
+
+The splat works on square tiles of 16, 32 or 64 pixels (one wave per 16 x 16
+quadrant); with ``tile_side='auto'`` (the default when the pixel is an integer
+number of PSF cells) the constructor times the three on the batch it was
+given and keeps the fastest (``tile_timings``, ``autotune_tiles()``).  Frames
+do not depend on the tile side.
 """
 import os
 
