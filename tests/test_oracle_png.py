@@ -105,3 +105,36 @@ def test_tirf_png_catches_swapped_draw_order(frame, png):
     adc = O.adc_counts(signal + noise, p.adc_fullwell, p.adc_gain, p.adc_offset, p.adc_bit)
     exact, worst, _ = _background_agreement(expected, adc, png)
     assert exact < 0.5, exact
+
+
+def _twocolor_frames():
+    """examples/twocolor.py: RandomState(123), 250 emitters, default
+    configuration (100 ms exposure); form_image on emitters [0, 200) then on
+    [150, 250) with the same RandomState, as the red and green channels."""
+    cfg = P.default_config()
+    rng = numpy.random.RandomState(123)
+    pl = cfg['detector']['pixel_length'] / cfg['magnification']
+    half = cfg['detector']['image_size'][0] * pl * 0.5
+    inputs = rng.uniform(-half, +half, size=(250, 2))
+    pre = {'scale': 1.0, 'origin': [0.0, 0, 0], 'unit_x': [1.0, 0, 0], 'unit_y': [0, 1.0, 0]}
+    frames, psf = [], None
+    for part in (inputs[:200], inputs[150:]):
+        p = P.epifm_params(cfg, rng)          # a new simulator per form_image call
+        psf = psf or O.PSFCache(p)            # same optics: the tables are reused
+        cam, _ = O.output_frame(p, psf, list(O.format_inputs(part, pre)), rng=rng)
+        frames.append(cam)
+    return frames
+
+
+def test_twocolor_png_background_pixels():
+    """Both channels of the reference's examples/twocolor_000.png: the second
+    frame only matches if the first consumed exactly the reference's number of
+    draws (noise vector, one choice uniform per pixel) from the shared
+    RandomState.  Blue is empty."""
+    png = numpy.load(os.path.join(GOLDEN, 'twocolor_000_u8.npz'))['image']
+    assert not png[:, :, 2].any()
+    for ch, cam in enumerate(_twocolor_frames()):
+        exact, worst, n_bg = _background_agreement(cam[:, :, 0], cam[:, :, 1], png[:, :, ch])
+        assert n_bg > 100_000, (ch, n_bg)
+        assert exact > 0.99, (ch, exact)
+        assert worst <= 1, (ch, worst)
