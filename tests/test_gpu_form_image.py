@@ -169,3 +169,37 @@ def test_filter_switches_follow_the_reference(gpu):
         scopyon_amd.form_image(inputs, config=product_config(ov), rng=numpy.random.RandomState(4))
     with pytest.raises(TypeError):
         _oracle_frames(ov, inputs, 4)
+
+
+def test_reference_example_images(gpu):
+    """The drop-in path run exactly as the reference's examples/tirf.py and
+    examples/twocolor.py (same calls, same RandomState(123)) reproduces their
+    committed output images (tests/golden/*_u8.npz, from examples/*.png) on
+    the background-only pixels: > 99% equal after the 8-bit rescale, all
+    within one grey level (the images predate the reference's current PSF
+    code, so pixels under emitters are not compared)."""
+    import os
+    import scopyon_amd
+    from test_oracle_png import GOLDEN, _background_agreement
+
+    def agree(img, info, png):
+        return _background_agreement(info['expectation'], img.as_array(), png)
+
+    config = scopyon_amd.DefaultConfiguration()
+    config.default.detector.exposure_time = 33.0e-3
+    pl = config.default.detector.pixel_length / config.default.magnification
+    L_2 = config.default.detector.image_size[0] * pl * 0.5
+    rng = numpy.random.RandomState(123)
+    inputs = rng.uniform(-L_2, +L_2, size=(100, 2))
+    img, info = scopyon_amd.form_image(inputs, config=config, rng=rng, full_output=True)
+    exact, worst, n_bg = agree(img, info, numpy.load(os.path.join(GOLDEN, 'tirf_000_u8.npz'))['image'])
+    assert n_bg > 150_000 and exact > 0.99 and worst <= 1, (exact, worst, n_bg)
+
+    config = scopyon_amd.DefaultConfiguration()
+    rng = numpy.random.RandomState(123)
+    inputs = rng.uniform(-L_2, +L_2, size=(250, 2))
+    png = numpy.load(os.path.join(GOLDEN, 'twocolor_000_u8.npz'))['image']
+    for ch, part in enumerate((inputs[: 200], inputs[150:])):
+        img, info = scopyon_amd.form_image(part, config=config, rng=rng, full_output=True)
+        exact, worst, n_bg = agree(img, info, png[:, :, ch])
+        assert n_bg > 100_000 and exact > 0.99 and worst <= 1, (ch, exact, worst, n_bg)
