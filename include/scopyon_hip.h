@@ -187,22 +187,6 @@ int smx_emccd_parity(int n, const double* E, const double* noise, const double* 
                      const smx_detector_params_t* params /* host */, double E_common,
                      const double* T_common, double* adc_out, double* signal_out,
                      int* status_dev, hipStream_t stream);
-/* Support length s_max - s_min of the EMCCD pmf for E (_epifm.py:372-376),
- * 0 for E <= 0 or E > 1e4 (host). */
-long long smx_emccd_support_len(double E, int emgain);
-/* The running pmf sums smx_emccd_parity's inverse-CDF scan forms for E, in
- * its order (cdf_out: device, at least smx_emccd_support_len(E, emgain)
- * doubles).  Lets every pixel sharing E (the background-only ones) find its
- * index by binary search instead of a scan: same draws. */
-int smx_emccd_common_cdf(double E, int emgain, double* cdf_out, long long cap,
-                         hipStream_t stream);
-/* smx_emccd_parity with those running sums for the pixels with E == E_common
- * (cdf_common from smx_emccd_common_cdf(E_common, ...); needs T_common). */
-int smx_emccd_parity_cdf(int n, const double* E, const double* noise, const double* uniforms,
-                         const int* uidx, const double* gain_map, const double* offset_map,
-                         const smx_detector_params_t* params /* host */, double E_common,
-                         const double* T_common, const double* cdf_common, double* adc_out,
-                         double* signal_out, int* status_dev, hipStream_t stream);
 /* EMCCD with Philox4x32-10 randomness: Poisson(E) -> Gamma(n, emgain) signal,
  * Gaussian readout noise, ADC.  Fuses _epifm.py:1436-1469 in float32
  * arithmetic (a statistical path; the parity path is smx_emccd_parity).  The Philox frame

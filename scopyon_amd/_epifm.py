@@ -449,17 +449,10 @@ class _EPIFMSimulator:
                                             else 0.0))
         T_common = _device.ARENA.typed('det_tcommon', 1, torch.float64, self.device)
         _lib.call('smx_emccd_common_total', E_common, self.det.emgain, T_common.data_ptr(), st)
-        # the background-only pixels share E_common: one set of running sums,
-        # searched per pixel (same draws as the per-pixel scan)
-        m = int(_lib.lib().smx_emccd_support_len(E_common, self.det.emgain))
-        cdf = None
-        if m > 0:
-            cdf = _device.ARENA.typed('det_cdf', m, torch.float64, self.device)
-            _lib.call('smx_emccd_common_cdf', E_common, self.det.emgain, cdf.data_ptr(), m, st)
         self.splatter.status.zero_()
-        _lib.call('smx_emccd_parity_cdf', n, E.data_ptr(), _device.ptr(noise), uni.data_ptr(),
+        _lib.call('smx_emccd_parity', n, E.data_ptr(), _device.ptr(noise), uni.data_ptr(),
                   uidx.data_ptr(), _device.ptr(gain_map), _device.ptr(offset_map), self.det,
-                  E_common, T_common.data_ptr(), _device.ptr(cdf), adc.data_ptr(), None,
+                  E_common, T_common.data_ptr(), adc.data_ptr(), None,
                   self.splatter.status.data_ptr(), st)
         return E, adc
 

@@ -84,11 +84,6 @@ SIGNATURES = {
     'smx_emccd_common_total': (c_int, [c_double, c_int, P, P]),
     'smx_emccd_parity': (c_int, [c_int, P, P, P, P, P, P, ctypes.POINTER(DetectorParamsT),
                                  c_double, P, P, P, P, P]),
-    'smx_emccd_parity_cdf': (c_int, [c_int, P, P, P, P, P, P,
-                                     ctypes.POINTER(DetectorParamsT), c_double, P, P, P, P, P,
-                                     P]),
-    'smx_emccd_support_len': (c_longlong, [c_double, c_int]),
-    'smx_emccd_common_cdf': (c_int, [c_double, c_int, P, c_longlong, P]),
     'smx_emccd_philox': (c_int, [c_int, P, P, P, ctypes.POINTER(DetectorParamsT), c_uint64,
                                  c_uint32, c_uint32, P, P, P, c_int, P]),
     'smx_brownian_predrawn': (c_int, [c_int, c_int, c_int, c_int, P, P, P, P, c_int, P, P, P, P,

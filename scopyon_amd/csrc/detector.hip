@@ -44,7 +44,7 @@ __device__ inline double emccd_term(long long S, double E, double a) {
     return 1.0 / Y * exp(-X + Y) * smx::i1e(Y);
 }
 
-SMX_HD void emccd_support(double E, int g, long long& s_min, long long& s_max) {
+__device__ inline void emccd_support(double E, int g, long long& s_min, long long& s_max) {
     double sigma = sqrt(E) * 5.0 + 10.0;
     long long lo = (long long)g * (long long)(E - sigma);   // int() truncates toward 0
     s_min = lo > 0 ? lo : 0;
@@ -83,37 +83,12 @@ __global__ void emccd_common_total_kernel(double E, int g, double* __restrict__ 
     if (lane == 0) *T_out = T;
 }
 
-// The pass-2 running sums for E: cdf[k] is the cumulative mass the parity
-// kernel's scan holds at support index k (same 64-term chunks, same in-wave
-// prefix order, same carry), so pixels sharing E can binary-search it and
-// draw the same index as the scan.
-__global__ void emccd_common_cdf_kernel(double E, int g, long long len, double* __restrict__ cdf) {
-    const int lane = threadIdx.x & 63;
-    const double a = 1.0 / (double)g;
-    long long s_min, s_max;
-    emccd_support(E, g, s_min, s_max);
-    double carry = 0.0;
-    for (long long c0 = 0; c0 < len; c0 += 64) {
-        const long long k = c0 + lane;
-        double v = k < len ? emccd_term(s_min + k, E, a) : 0.0;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            double o = __shfl_up(v, off, 64);
-            if (lane >= off) v += o;
-        }
-        const double cum = carry + v;
-        if (k < len) cdf[k] = cum;
-        carry = __shfl(cum, 63, 64);
-    }
-}
-
 __global__ void __launch_bounds__(256) emccd_parity_kernel(
         int n, const double* __restrict__ E_arr, const double* __restrict__ noise,
         const double* __restrict__ uniforms, const int* __restrict__ uidx,
         const double* __restrict__ gain_map, const double* __restrict__ offset_map,
         smx_detector_params_t P, double E_common, const double* __restrict__ T_common,
-        const double* __restrict__ cdf_common, double* __restrict__ adc_out,
-        double* __restrict__ signal_out, int* __restrict__ status) {
+        double* __restrict__ adc_out, double* __restrict__ signal_out, int* __restrict__ status) {
     const int lane = threadIdx.x & 63;
     const int pix = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (pix >= n) return;
@@ -133,17 +108,6 @@ __global__ void __launch_bounds__(256) emccd_parity_kernel(
         if (!(T > 0.0) || !(T < INFINITY)) {
             // the reference's pmf overflows here (numpy.random.choice raises ValueError)
             if (lane == 0) atomicOr(status, 4);
-        } else if (cdf_common && T_common && E == E_common) {
-            // the shared running sums: first k with cdf[k] > thr (else len - 1),
-            // the index the scan below would stop at
-            const double thr = uniforms[uidx[pix]] * T;
-            long long lo = 0, hi = len;
-            while (lo < hi) {
-                const long long mid = (lo + hi) >> 1;
-                if (cdf_common[mid] > thr) hi = mid;
-                else lo = mid + 1;
-            }
-            S = (double)(s_min + (lo < len ? lo : len - 1));
         } else {
             const double thr = uniforms[uidx[pix]] * T;
             double carry = 0.0;
@@ -625,12 +589,11 @@ int smx_emccd_common_total(double E, int emgain, double* T_out, hipStream_t stre
     return SMX_OK;
 }
 
-static int emccd_parity_launch(int n, const double* E, const double* noise,
-                               const double* uniforms, const int* uidx, const double* gain_map,
-                               const double* offset_map, const smx_detector_params_t* params,
-                               double E_common, const double* T_common,
-                               const double* cdf_common, double* adc_out, double* signal_out,
-                               int* status_dev, hipStream_t stream) {
+int smx_emccd_parity(int n, const double* E, const double* noise, const double* uniforms,
+                     const int* uidx, const double* gain_map, const double* offset_map,
+                     const smx_detector_params_t* params, double E_common,
+                     const double* T_common, double* adc_out, double* signal_out,
+                     int* status_dev, hipStream_t stream) {
     if (n < 0 || !params || params->emgain <= 0 ||
         (n > 0 && (!E || !uniforms || !uidx || !adc_out || !status_dev))) {
         smx::set_error("smx_emccd_parity: invalid arguments (n=%d)", n);
@@ -638,50 +601,9 @@ static int emccd_parity_launch(int n, const double* E, const double* noise,
     }
     if (n == 0) return SMX_OK;
     hipLaunchKernelGGL(emccd_parity_kernel, dim3((n + 3) / 4), dim3(256), 0, stream, n, E, noise,
-                       uniforms, uidx, gain_map, offset_map, *params, E_common, T_common,
-                       cdf_common, adc_out, signal_out, status_dev);
+                       uniforms, uidx, gain_map, offset_map, *params, E_common, T_common, adc_out,
+                       signal_out, status_dev);
     SMX_CHECK_LAUNCH("emccd_parity_kernel");
-    return SMX_OK;
-}
-
-int smx_emccd_parity(int n, const double* E, const double* noise, const double* uniforms,
-                     const int* uidx, const double* gain_map, const double* offset_map,
-                     const smx_detector_params_t* params, double E_common,
-                     const double* T_common, double* adc_out, double* signal_out,
-                     int* status_dev, hipStream_t stream) {
-    return emccd_parity_launch(n, E, noise, uniforms, uidx, gain_map, offset_map, params,
-                               E_common, T_common, nullptr, adc_out, signal_out, status_dev,
-                               stream);
-}
-
-int smx_emccd_parity_cdf(int n, const double* E, const double* noise, const double* uniforms,
-                         const int* uidx, const double* gain_map, const double* offset_map,
-                         const smx_detector_params_t* params, double E_common,
-                         const double* T_common, const double* cdf_common, double* adc_out,
-                         double* signal_out, int* status_dev, hipStream_t stream) {
-    return emccd_parity_launch(n, E, noise, uniforms, uidx, gain_map, offset_map, params,
-                               E_common, T_common, cdf_common, adc_out, signal_out, status_dev,
-                               stream);
-}
-
-long long smx_emccd_support_len(double E, int emgain) {
-    if (!(E > 0.0) || !(E <= 1.0e4) || emgain <= 0) return 0;
-    long long s_min, s_max;
-    emccd_support(E, emgain, s_min, s_max);
-    return s_max > s_min ? s_max - s_min : 0;
-}
-
-int smx_emccd_common_cdf(double E, int emgain, double* cdf_out, long long cap, hipStream_t stream) {
-    const long long len = smx_emccd_support_len(E, emgain);
-    if (emgain <= 0 || (len > 0 && (!cdf_out || cap < len))) {
-        smx::set_error("smx_emccd_common_cdf: invalid arguments (emgain=%d, support %lld, "
-                       "capacity %lld)", emgain, len, cap);
-        return SMX_EVALUE;
-    }
-    if (len == 0) return SMX_OK;
-    hipLaunchKernelGGL(emccd_common_cdf_kernel, dim3(1), dim3(64), 0, stream, E, emgain, len,
-                       cdf_out);
-    SMX_CHECK_LAUNCH("emccd_common_cdf_kernel");
     return SMX_OK;
 }
 

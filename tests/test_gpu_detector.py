@@ -185,50 +185,3 @@ def test_philox_poisson_sampler_distribution(gpu, lam):
     chi2 = float(((obs - exp) ** 2 / exp).sum())
     pval = stats.chi2.sf(chi2, len(exp) - 1)
     assert pval > 1e-4, (lam, chi2, len(exp), pval)
-
-
-def test_emccd_parity_shared_cdf_equals_scan(gpu):
-    """Pixels sharing E_common searched in smx_emccd_common_cdf's running sums
-    draw exactly what the per-pixel scan draws (and what the oracle draws),
-    including uniforms at 0, just below 1 and exactly on a running sum."""
-    from scopyon_amd import _lib, _device
-    p = oracle_params({'detector': {'image_size': [64, 64]}})
-    det = _det_params(p)
-    rng = numpy.random.RandomState(9)
-    E_common = float(p.QE * p.background_mean)
-    E = numpy.full((64, 64), E_common)
-    spots = rng.random_sample(E.shape) < 0.3
-    E[spots] = rng.uniform(0.5, 60.0, spots.sum())
-    n = E.size
-    m = int(_lib.lib().smx_emccd_support_len(E_common, det.emgain))
-    assert m > 64
-    T = torch.zeros(1, dtype=torch.float64, device=gpu)
-    cdf = torch.zeros(m, dtype=torch.float64, device=gpu)
-    _lib.call('smx_emccd_common_total', E_common, det.emgain, T.data_ptr(), _device.stream())
-    _lib.call('smx_emccd_common_cdf', E_common, det.emgain, cdf.data_ptr(), m, _device.stream())
-    c = cdf.cpu().numpy()
-    assert (numpy.diff(c) >= 0).all() and abs(c[-1] / T.item() - 1.0) < 1e-12
-    u = rng.random_sample(n)
-    bg = numpy.flatnonzero(~spots.ravel())
-    u[bg[:4]] = [0.0, 1.0 - 2.0 ** -53, c[m // 3] / T.item(), c[10] / T.item()]
-    noise = rng.normal(0.0, p.readout_noise, n)
-    dE, du, dn = (_device.upload(a, device=gpu) for a in (E.ravel(), u, noise))
-    uidx = _device.upload(numpy.arange(n, dtype=numpy.int32), dtype=torch.int32, device=gpu)
-    outs = []
-    for entry in ('smx_emccd_parity', 'smx_emccd_parity_cdf'):
-        adc = torch.empty(n, dtype=torch.float64, device=gpu)
-        sig = torch.empty(n, dtype=torch.float64, device=gpu)
-        status = torch.zeros(1, dtype=torch.int32, device=gpu)
-        extra = (cdf.data_ptr(),) if entry.endswith('_cdf') else ()
-        _lib.call(entry, n, dE.data_ptr(), dn.data_ptr(), du.data_ptr(), uidx.data_ptr(), None,
-                  None, det, E_common, T.data_ptr(), *extra, adc.data_ptr(), sig.data_ptr(),
-                  status.data_ptr(), _device.stream())
-        assert int(status.item()) == 0
-        outs.append((adc.cpu().numpy(), sig.cpu().numpy()))
-    assert numpy.array_equal(outs[0][0], outs[1][0]) and numpy.array_equal(outs[0][1], outs[1][1])
-    # the oracle (numpy's cumsum order) agrees except where u sits exactly on
-    # one of the device's running sums: the two planted ties
-    want, _ = _oracle(p, E, noise.reshape(E.shape), u)
-    keep = numpy.ones(n, bool)
-    keep[bg[2:4]] = False
-    numpy.testing.assert_array_equal(outs[1][0][keep], want.ravel()[keep])
