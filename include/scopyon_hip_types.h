@@ -15,7 +15,9 @@ extern "C" {
  * image_size[1] (v) (src/scopyon/_epifm.py:1175-1177). */
 typedef struct smx_fov_t {
     int32_t nw, nh;          /* image_size = (nw, nh) */
-    int32_t tiles_r, tiles_c;/* ceil(nw / 32), ceil(nh / 32): smx_splat_tile_size() */
+    int32_t tiles_r, tiles_c;/* ceil(nw / t), ceil(nh / t): tile side t = 32
+                                (smx_splat_tile_size()), or the side given to
+                                smx_splat_*_ts */
     int64_t pix_off;         /* first pixel of this frame in the batch buffer */
     int32_t tile_off;        /* first global tile id of this frame */
     int32_t reserved;
