@@ -46,18 +46,11 @@ constexpr float INV_SQRT2 = 0.70710678118654752f;
 constexpr float INV_SQRT2PI = 0.39894228040143268f;
 
 // Pixel-integrated unit Gaussian along one axis: (erf(t1) - erf(t0)) / 2 with
-// t = (edge - x) / (sqrt(2) sigma); the complementary form keeps the tails
-// accurate (no difference of two numbers close to +-1).
-__device__ inline float g_pixel(float i, float x, float s) {
-    float t0 = (i - x) * s, t1 = (i + 1.0f - x) * s;
-    if (t0 >= 0.0f) return 0.5f * (erfcf(t0) - erfcf(t1));
-    if (t1 <= 0.0f) return 0.5f * (erfcf(-t1) - erfcf(-t0));
-    return 0.5f * (erff(t1) - erff(t0));
-}
-
-// g_pixel from the pixel's edge arguments t0 < t1 and e0 = erfc(|t0|),
-// e1 = erfc(|t1|): the same value as g_pixel (the straddling pixel takes the
-// erf branch itself).
+// t = (edge - x) / (sqrt(2) sigma) for the pixel's two edges t0 < t1.  A pixel
+// on one side of x takes the complementary form from e0 = erfc(|t0|) and
+// e1 = erfc(|t1|), which keeps the tails accurate (no difference of two
+// numbers close to +-1); the pixel containing x takes erf itself.  The edge
+// values are shared with the neighbouring pixels by the callers.
 __device__ inline float g_pixel_edges(float t0, float t1, float e0, float e1) {
     if (t0 >= 0.0f) return 0.5f * (e0 - e1);
     if (t1 <= 0.0f) return 0.5f * (e1 - e0);
@@ -244,7 +237,11 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
             const int ax = r >= W, k = ax ? r - W : r;
             const float pos = ax ? s_xy[c].y : s_xy[c].x;
             const int lo = ax ? s_lo[c].y : s_lo[c].x;
-            s_g[c][ax][k] = g_pixel((float)(lo + k), pos, s);
+            // two erfc per entry and no three-way divergence: the erfc values
+            // serve both one-sided cases, the straddling pixel (at most one
+            // per axis and emitter) takes erf itself
+            const float t0 = ((float)(lo + k) - pos) * s, t1 = ((float)(lo + k) + 1.0f - pos) * s;
+            s_g[c][ax][k] = g_pixel_edges(t0, t1, erfcf(fabsf(t0)), erfcf(fabsf(t1)));
         }
         __syncthreads();
         if (active) {
