@@ -274,22 +274,21 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
             for (int b = 0; b < total; b += NSLOT) {
                 float val[NSLOT];
                 int pix[NSLOT];
+                // branch-free part decoding (24-bit multiplies; factor indices
+                // clamped into their 16-entry rows, masked lanes add nothing),
+                // so the LDS reads of the NSLOT parts issue back to back
 #pragma unroll
                 for (int q = 0; q < NSLOT; ++q) {
-                    pix[q] = -1;
-                    val[q] = 0.0f;
-                    if (b + q < total) {
-                        const int ent = s_part[wave][b + q], c = ent >> 2;
-                        const int4 Q = s_qi[wave][c];
-                        const int idx = (ent & 3) * 64 + lane;
-                        if (idx < Q.z) {
-                            const int Wr = Q.y & 31, magic = Q.y >> 5;
-                            const int row = (idx * magic) >> 16, col = idx - row * Wr;
-                            const int ru = (Q.w & 255) + row, cv = (Q.w >> 8) + col;
-                            val[q] = (s_amp[c] * s_g[c][0][ru]) * s_g[c][1][cv];
-                            pix[q] = Q.x + row * QS + col;
-                        }
-                    }
+                    const int ent = s_part[wave][min(b + q, total - 1)], c = ent >> 2;
+                    const int4 Q = s_qi[wave][c];
+                    const int idx = (ent & 3) * 64 + lane;
+                    const bool ok = (b + q < total) & (idx < Q.z);
+                    const int Wr = Q.y & 31, magic = Q.y >> 5;
+                    const int row = (int)(__umul24((unsigned)idx, (unsigned)magic) >> 16);
+                    const int col = idx - __mul24(row, Wr);
+                    const int ru = min((Q.w & 255) + row, 15), cv = min((Q.w >> 8) + col, 15);
+                    val[q] = (s_amp[c] * s_g[c][0][ru]) * s_g[c][1][cv];
+                    pix[q] = ok ? Q.x + row * QS + col : -1;
                 }
 #pragma unroll
                 for (int q = 0; q < NSLOT; ++q)
