@@ -85,36 +85,47 @@ __device__ inline int frame_of(const int* __restrict__ frame_id, int i) { return
 // atomic per block and tile; a window of BIN_WIN tiles from the first tile of
 // the block's first frame).  A window of 2R + 1 <= 15 pixels spans at most
 // 2 x 2 tiles.
+// EPT emitters per thread (coalesced: emitter base + e * BIN_THREADS + tid):
+// with one frame of many tiles (1 M emitters on 2048^2) a block's emitters
+// spread over most of its window, and more of them per block means fewer
+// device atomics per entry.
 __device__ inline int gauss_window_base(const int* __restrict__ frame_id,
-                                        const smx_fov_t* __restrict__ frames, int nframes, int n) {
-    const int i0 = blockIdx.x * smx::BIN_THREADS;
-    const int f0 = i0 < n ? frame_of(frame_id, i0) : -1;
+                                        const smx_fov_t* __restrict__ frames, int nframes, int n,
+                                        int ept) {
+    const long long i0 = (long long)blockIdx.x * smx::BIN_THREADS * ept;
+    const int f0 = i0 < n ? frame_of(frame_id, (int)i0) : -1;
     return f0 >= 0 && f0 < nframes ? frames[f0].tile_off : 0;
 }
 
+template <int EPT>
 __global__ void __launch_bounds__(smx::BIN_THREADS) gauss_bin_count_kernel(
         int n, const float* __restrict__ x, const float* __restrict__ y,
         const int* __restrict__ frame_id, const smx_fov_t* __restrict__ frames, int nframes, int R,
         int n_tiles, int* __restrict__ tile_count, int cs) {
     __shared__ int s_hist[smx::BIN_WIN];
     smx::bin_hist_clear(s_hist);
-    const int wbase = gauss_window_base(frame_id, frames, nframes, n);
+    const int wbase = gauss_window_base(frame_id, frames, nframes, n, EPT);
     __syncthreads();
-    const int i = blockIdx.x * smx::BIN_THREADS + threadIdx.x;
-    const int f = i < n ? frame_of(frame_id, i) : -1;
-    TileRange t;
-    if (f >= 0 && f < nframes && window_tiles(frames[f], x[i], y[i], R, t)) {
-        const smx_fov_t& F = frames[f];
-        for (int r = t.r0; r <= t.r1; ++r)
-            for (int c = t.c0; c <= t.c1; ++c) {
-                const int tile = F.tile_off + r * F.tiles_c + c;
-                if (tile >= 0 && tile < n_tiles) smx::bin_count(s_hist, wbase, tile, tile_count, cs);
-            }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const long long i = ((long long)blockIdx.x * EPT + e) * smx::BIN_THREADS + threadIdx.x;
+        const int f = i < n ? frame_of(frame_id, (int)i) : -1;
+        TileRange t;
+        if (f >= 0 && f < nframes && window_tiles(frames[f], x[i], y[i], R, t)) {
+            const smx_fov_t& F = frames[f];
+            for (int r = t.r0; r <= t.r1; ++r)
+                for (int c = t.c0; c <= t.c1; ++c) {
+                    const int tile = F.tile_off + r * F.tiles_c + c;
+                    if (tile >= 0 && tile < n_tiles)
+                        smx::bin_count(s_hist, wbase, tile, tile_count, cs);
+                }
+        }
     }
     __syncthreads();
     smx::bin_count_flush(s_hist, wbase, tile_count, cs);
 }
 
+template <int EPT>
 __global__ void __launch_bounds__(smx::BIN_THREADS) gauss_bin_fill_kernel(
         int n, const float* __restrict__ x, const float* __restrict__ y,
         const int* __restrict__ frame_id, const smx_fov_t* __restrict__ frames, int nframes, int R,
@@ -122,44 +133,51 @@ __global__ void __launch_bounds__(smx::BIN_THREADS) gauss_bin_fill_kernel(
         int* __restrict__ list, long long capacity, int* __restrict__ status) {
     __shared__ int s_hist[smx::BIN_WIN];
     smx::bin_hist_clear(s_hist);
-    const int wbase = gauss_window_base(frame_id, frames, nframes, n);
+    const int wbase = gauss_window_base(frame_id, frames, nframes, n, EPT);
     __syncthreads();
-    const int i = blockIdx.x * smx::BIN_THREADS + threadIdx.x;
-    int ent[4];   // window entries: tile offset << 16 | LDS rank (-1: none)
+    int ent[EPT][4];   // window entries: tile offset << 16 | LDS rank (-1: none)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ent[q] = -1;
-    const int f = i < n ? frame_of(frame_id, i) : -1;
-    TileRange t;
-    if (f >= 0 && f < nframes && window_tiles(frames[f], x[i], y[i], R, t)) {
-        const smx_fov_t& F = frames[f];
-        int q = 0;
-        for (int r = t.r0; r <= t.r1; ++r)
-            for (int c = t.c0; c <= t.c1; ++c) {
-                const int tile = F.tile_off + r * F.tiles_c + c;
-                if (tile < 0 || tile >= n_tiles) continue;
-                const unsigned d = (unsigned)(tile - wbase);
-                if (d < (unsigned)smx::BIN_WIN) {
-                    // ranks stay below 2^16: at most BIN_THREADS entries per tile
-                    ent[q & 3] = (int)(d << 16) | atomicAdd(&s_hist[d], 1);
-                    ++q;
-                } else {
-                    const int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;
-                    const long long at = (long long)tile_begin[tile] + pos;
-                    if (at < capacity) list[at] = i;
-                    else atomicOr(status, 2);
+    for (int e = 0; e < EPT; ++e) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ent[e][q] = -1;
+        const long long i = ((long long)blockIdx.x * EPT + e) * smx::BIN_THREADS + threadIdx.x;
+        const int f = i < n ? frame_of(frame_id, (int)i) : -1;
+        TileRange t;
+        if (f >= 0 && f < nframes && window_tiles(frames[f], x[i], y[i], R, t)) {
+            const smx_fov_t& F = frames[f];
+            int q = 0;
+            for (int r = t.r0; r <= t.r1; ++r)
+                for (int c = t.c0; c <= t.c1; ++c) {
+                    const int tile = F.tile_off + r * F.tiles_c + c;
+                    if (tile < 0 || tile >= n_tiles) continue;
+                    const unsigned d = (unsigned)(tile - wbase);
+                    if (d < (unsigned)smx::BIN_WIN) {
+                        // ranks stay below 2^16: at most EPT * BIN_THREADS entries per tile
+                        ent[e][q & 3] = (int)(d << 16) | atomicAdd(&s_hist[d], 1);
+                        ++q;
+                    } else {
+                        const int pos = atomicSub(&tile_left[(size_t)tile * cs], 1) - 1;
+                        const long long at = (long long)tile_begin[tile] + pos;
+                        if (at < capacity) list[at] = (int)i;
+                        else atomicOr(status, 2);
+                    }
                 }
-            }
+        }
     }
     __syncthreads();
     smx::bin_claim(s_hist, wbase, tile_left, cs);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (ent[q] < 0) continue;
-        const int d = ent[q] >> 16, rank = ent[q] & 0xffff;
-        const long long at = (long long)tile_begin[wbase + d] + s_hist[d] + rank;
-        if (at < capacity) list[at] = i;
-        else atomicOr(status, 2);
+    for (int e = 0; e < EPT; ++e) {
+        const int i = (blockIdx.x * EPT + e) * smx::BIN_THREADS + threadIdx.x;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (ent[e][q] < 0) continue;
+            const int d = ent[e][q] >> 16, rank = ent[e][q] & 0xffff;
+            const long long at = (long long)tile_begin[wbase + d] + s_hist[d] + rank;
+            if (at < capacity) list[at] = i;
+            else atomicOr(status, 2);
+        }
     }
 }
 
@@ -725,14 +743,23 @@ namespace {
 int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
               const smx_fov_t* frames_dev, int nframes, int n_tiles, int R, long long capacity,
               GaussWorkspace& ws, int* status_dev, hipStream_t stream) {
-    const int nbb = (n + smx::BIN_THREADS - 1) / smx::BIN_THREADS;
+    // four emitters per thread for large launches (denser LDS windows)
+    const int ept = n >= (1 << 19) ? 4 : 1;
+    const int nbb = (int)(((long long)n + (long long)smx::BIN_THREADS * ept - 1) /
+                          ((long long)smx::BIN_THREADS * ept));
     const int cs = smx::count_stride(2LL * n, n_tiles);
     hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
                        stream, nframes, frames_dev, n_tiles, ws.tile_count, cs, ws.tile_fov);
     SMX_CHECK_LAUNCH("tiles_init_kernel");
     if (n > 0) {
-        hipLaunchKernelGGL(gauss_bin_count_kernel, dim3(nbb), dim3(smx::BIN_THREADS), 0, stream, n,
-                           x, y, frame_id, frames_dev, nframes, R, n_tiles, ws.tile_count, cs);
+        if (ept == 4)
+            hipLaunchKernelGGL(gauss_bin_count_kernel<4>, dim3(nbb), dim3(smx::BIN_THREADS), 0,
+                               stream, n, x, y, frame_id, frames_dev, nframes, R, n_tiles,
+                               ws.tile_count, cs);
+        else
+            hipLaunchKernelGGL(gauss_bin_count_kernel<1>, dim3(nbb), dim3(smx::BIN_THREADS), 0,
+                               stream, n, x, y, frame_id, frames_dev, nframes, R, n_tiles,
+                               ws.tile_count, cs);
         SMX_CHECK_LAUNCH("gauss_bin_count_kernel");
     }
     size_t cb = ws.cub_bytes;
@@ -744,10 +771,14 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
                        "hipcub ExclusiveSum"))
         return SMX_ERUNTIME;
     if (n > 0) {
-        hipLaunchKernelGGL(gauss_bin_fill_kernel, dim3(nbb), dim3(smx::BIN_THREADS), 0, stream, n,
-                           x, y,
-                           frame_id, frames_dev, nframes, R, n_tiles, ws.tile_begin,
-                           ws.tile_count, cs, ws.list, capacity, status_dev);
+        if (ept == 4)
+            hipLaunchKernelGGL(gauss_bin_fill_kernel<4>, dim3(nbb), dim3(smx::BIN_THREADS), 0,
+                               stream, n, x, y, frame_id, frames_dev, nframes, R, n_tiles,
+                               ws.tile_begin, ws.tile_count, cs, ws.list, capacity, status_dev);
+        else
+            hipLaunchKernelGGL(gauss_bin_fill_kernel<1>, dim3(nbb), dim3(smx::BIN_THREADS), 0,
+                               stream, n, x, y, frame_id, frames_dev, nframes, R, n_tiles,
+                               ws.tile_begin, ws.tile_count, cs, ws.list, capacity, status_dev);
         SMX_CHECK_LAUNCH("gauss_bin_fill_kernel");
     }
     return SMX_OK;

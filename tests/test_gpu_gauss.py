@@ -250,6 +250,37 @@ def test_separable_forms_against_oracle(gpu, method, rms_tol, max_tol):
         assert rms <= rms_tol and mx <= max_tol, (method, f, rms, mx)
 
 
+@pytest.mark.parametrize('method', ['lds', 'mfma-f32'])
+def test_large_launch_binning_against_oracle(gpu, method):
+    """600,000 emitters on one 1024^2 frame: the binning takes four emitters
+    per thread (launches of 2^19 emitters and more) and tile lists of ~1,300
+    entries; a 48 x 48 crop equals the oracle (computed from the emitters whose
+    window reaches it) and the frame is deterministic."""
+    from scopyon_amd.gauss import GaussianModel
+    S, N, R, C = 1024, 600_000, 6, 48
+    rng = numpy.random.RandomState(8)
+    x, y = rng.uniform(0, S, N), rng.uniform(0, S, N)
+    I = rng.exponential(500.0, N)
+    kw = dict(sigma=1.2, inv_depth=0.0, background=0.0, radius=R)
+    m = GaussianModel([(S, S)], method=method, **kw)
+    xd, yd, Id = _dev(x), _dev(y), _dev(I)
+    zt = _dev(numpy.zeros(N)) if method == 'lds' else None
+    mu = m.forward(xd, yd, zt, Id)
+    mu2 = m.forward(xd, yd, zt, Id).clone()
+    assert m.check_status() & ~1 == 0
+    assert torch.equal(mu, mu2)
+    x32, y32, I32 = (a.astype(numpy.float32).astype(float) for a in (x, y, I))
+    c0 = 500
+    sel = ((x32 >= c0 - R - 1) & (x32 < c0 + C + R + 1) & (y32 >= c0 - R - 1) &
+           (y32 < c0 + C + R + 1))
+    k = int(sel.sum())
+    ref = G.render([(C, C)], x32[sel] - c0, y32[sel] - c0, numpy.zeros(k), I32[sel],
+                   numpy.zeros(k, int), **kw)[0]
+    got = mu.view(S, S)[c0:c0 + C, c0:c0 + C].cpu().numpy().astype(float)
+    rms = numpy.linalg.norm(got - ref) / numpy.linalg.norm(ref)
+    assert rms <= 2e-6, (method, rms)
+
+
 def test_mfma_forward_deterministic_and_unsorted_lists(gpu):
     """Repeatable bits for sorted tile lists; a tile list longer than the LDS
     sort keeps the image (summation order aside) and reports status bit 1."""
