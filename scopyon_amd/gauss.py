@@ -32,7 +32,7 @@ class GaussianModel:
     frames of possibly different sizes."""
 
     def __init__(self, frame_shapes, *, sigma=1.2, inv_depth=1.0 / 0.377, background=10.0,
-                 radius=6, method='lds', device=None):
+                 radius=6, method='mfma-f32', device=None):
         self.device = device or _device.require_gpu()
         if method not in self.METHODS:
             raise ValueError('method must be one of {}'.format(sorted(self.METHODS)))
@@ -110,10 +110,11 @@ class GaussianModel:
     METHODS = {'lds': None, 'mfma-f32': 0, 'mfma-f16': 1, 'mfma-bf16': 2}
 
     def forward(self, x, y, z, I, frame_id=None, sigma=None, out=None, method=None):
-        """Expected image.  method: 'lds' (float32 products added in LDS, the
-        default) or 'mfma-f32' / 'mfma-f16' / 'mfma-bf16' (the separable
-        contraction on the matrix cores with float32, float16 or bfloat16
-        operands and float32 accumulation).  z may be None with the MFMA forms."""
+        """Expected image.  method: 'mfma-f32' (the default) / 'mfma-f16' /
+        'mfma-bf16' (the separable contraction on the matrix cores with
+        float32, float16 or bfloat16 operands and float32 accumulation; z may
+        be None) or 'lds' (float32 products added in LDS; same accuracy as
+        'mfma-f32', slower at these densities)."""
         method = method or self.method
         if method not in self.METHODS:
             raise ValueError('method must be one of {}'.format(sorted(self.METHODS)))

@@ -32,15 +32,16 @@ def _dev(a, dtype=torch.float32):
     return torch.as_tensor(numpy.ascontiguousarray(a), dtype=dtype, device='cuda')
 
 
-def _model(shapes):
+def _model(shapes, method='mfma-f32'):
     from scopyon_amd.gauss import GaussianModel
-    return GaussianModel(shapes, **KW)
+    return GaussianModel(shapes, method=method, **KW)
 
 
-def test_forward_and_nll_match_oracle(gpu):
+@pytest.mark.parametrize('method', ['mfma-f32', 'lds'])
+def test_forward_and_nll_match_oracle(gpu, method):
     shapes = [(64, 64), (40, 72), (100, 33)]
     x, y, z, I, fid, k = _case(0, shapes, 60)
-    m = _model(shapes)
+    m = _model(shapes, method)
     mu = m.forward(_dev(x), _dev(y), _dev(z), _dev(I), _dev(fid, torch.int32))
     assert m.check_status() & ~1 == 0
     ref = G.render(shapes, x, y, z, I, fid, **KW)
@@ -78,10 +79,11 @@ def test_many_frames_span_the_binning_window(gpu, method):
                                       err_msg=str(f))
 
 
-def test_forward_is_deterministic(gpu):
+@pytest.mark.parametrize('method', ['mfma-f32', 'lds'])
+def test_forward_is_deterministic(gpu, method):
     shapes = [(96, 96)]
     x, y, z, I, fid, _ = _case(1, shapes, 400)
-    m = _model(shapes)
+    m = _model(shapes, method)
     args = [_dev(x), _dev(y), _dev(z), _dev(I), None]
     a = m.forward(*args).clone()
     b = m.forward(*args).clone()
