@@ -16,6 +16,8 @@
 // contributions in emitter order, like the reference loop, and the result is
 // run-to-run deterministic), then one 256-thread workgroup per tile
 // accumulates its 1024 pixels in registers.  No float atomics.
+#include <cstdlib>
+#include <type_traits>
 #include "common.hpp"
 #include "scopyon_hip_types.h"
 #include "tiles.hpp"
@@ -695,33 +697,68 @@ __device__ __forceinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& 
 }
 
 // One wave per 16 x 16 quadrant of a tile of QPW x QPW quadrants (side 16,
-// 32 or 64: the tile shape the batched path's autotuner picks).  Per chunk of the sorted
-// tile list, each wave lists the 64-pixel parts of the emitters' footprints
-// that fall in its quadrant (lane = pixel of the footprint rectangle, so no
-// lane idles on pixels outside it), issues the table loads of NSLOT parts at
-// once, then adds the contributions into its LDS accumulator part by part in
-// list order: every pixel still sums its emitters in emitter order, and the
-// lanes of one part touch distinct pixels.  KT is the phase-table element
-// type: double (values identical to the SAT path) or float (each contribution
-// rounded to float32, relative error <= 2^-24).
+// 32 or 64: the tile shape the batched path's autotuner picks).  Per chunk of
+// the sorted tile list, each wave clips the emitters' footprints to its
+// quadrant and splits each rectangle into 64-pixel parts (lane = pixel of the
+// rectangle, so no lane idles outside it).  The parts go to LDS as decoded
+// records, up to PMAX per round, padded with null records to whole groups of
+// NSLOT; per group the lanes issue NSLOT table loads at once (24-bit
+// multiply-adds on uniform record fields, byte offsets from the table base),
+// then add the contributions into the wave's LDS accumulator part by part in
+// list order, with no branch: lanes outside a part add into their own spill
+// slot.  Every pixel sums its emitters in emitter order, and the lanes of one
+// part touch distinct pixels.  KT is the phase-table element type: double
+// (values identical to the SAT path) or float (each contribution rounded to
+// float32, relative error <= 2^-24).
 constexpr int QS = 16;            // quadrant side
+// Accumulator row stride (doubles).  ds_add_f64 serves 32 lanes per 256 bytes
+// of banks; with 16 the rows of a narrow part share banks, 17 spreads them
+// (measured: 17 is the fastest of 16, 17, 18, 20 and 24 for 32-pixel tiles).
+#ifndef SMX_ACC_STRIDE
+#define SMX_ACC_STRIDE 17
+#endif
+constexpr int AS = SMX_ACC_STRIDE;
 constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
 // tile list sorted in LDS: 2048 entries, 1024 for one-wave (16 x 16) tiles
 template <int QPW> constexpr int psort_cap() { return QPW == 1 ? 1024 : 2048; }
 
-template <typename KT, int QPW>
-__global__ void __launch_bounds__(64 * QPW * QPW) splat_phase_kernel(FastArgs A) {
+// One 64-pixel part of an emitter's footprint rectangle inside a quadrant,
+// decoded for the lanes: pixel idx = p64 + lane of the rectangle (row-major,
+// width W) is valid for lane < lim; row = (idx * magic) >> 16 (exact for
+// idx < 256, W <= 16); its table element is kq + row * su + col * sv (bytes)
+// and its accumulator slot pix + row * AS + col (bytes * 8).
+struct __align__(16) PartRec {
+    int kq;          // table byte offset (32-bit form) or low half of the byte address
+    int magic_lim;   // ceil(65536 / W) | lim << 24 (valid lanes, 0..64)
+    int negw;        // -W
+    int p64;         // first rectangle pixel of the part
+    int su, sv;      // table bytes per pixel along u, v
+    int pix;         // accumulator byte offset of the rectangle's first pixel
+    int kq_hi;       // high half of the byte address (64-bit form)
+};
+constexpr int PMAX = 64;          // parts per wave per round (one lane writes each padding record)
+// waves per SIMD the register allocation leaves room for (5 at 256 threads:
+// the LDS of 5 blocks fits a CU)
+#ifndef SMX_SPLAT_WPE
+#define SMX_SPLAT_WPE 5
+#endif
+
+template <typename KT, int QPW, bool O32>
+__global__ void __launch_bounds__(64 * QPW * QPW) __attribute__((amdgpu_waves_per_eu(SMX_SPLAT_WPE, 8)))
+splat_phase_kernel(FastArgs A) {
     constexpr int WPT = QPW * QPW, TSZ = QS * QPW, PSORT_CAP = psort_cap<QPW>();
+    constexpr int ACC = QS * AS;
+    // accumulators, then one spill slot per lane shared by the waves (lanes
+    // outside a part add there: no branch per part, the slots are never read)
+    __shared__ double s_acc[WPT * ACC + 64];
     __shared__ int s_sort[PSORT_CAP];
-    __shared__ double s_acc[WPT][QS * QS];
     __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (frame pixels), n_u, n_v
     __shared__ int2 s_str[FCHUNK];            // table strides per pixel along u, v
     __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
     __shared__ double s_w[FCHUNK];
-    __shared__ const KT* s_kb[FCHUNK];        // table element of footprint pixel (0, 0)
-    __shared__ int4 s_qi[WPT][FCHUNK];        // per wave: footprint rectangle in the quadrant
-    __shared__ const KT* s_qk[WPT][FCHUNK];   // per wave: its first table element
-    __shared__ int s_part[WPT][FCHUNK * 4];   // per wave: emitter << 2 | part
+    __shared__ long long s_ko[FCHUNK];        // table element of footprint pixel (0, 0)
+    __shared__ PartRec s_pr[WPT][PMAX];       // per wave: the parts of one round
+    __shared__ double s_pw[WPT][PMAX];        // and their emitter weights
 
     const int tile = blockIdx.x;
     const int fov = A.tile_fov[tile];
@@ -729,12 +766,16 @@ __global__ void __launch_bounds__(64 * QPW * QPW) splat_phase_kernel(FastArgs A)
     const smx_fov_t F = A.fovs[fov];
     const int local = tile - F.tile_off;
     const int tr = local / F.tiles_c, tc = local % F.tiles_c;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int qr0 = tr * TSZ + (wave / QPW) * QS, qc0 = tc * TSZ + (wave % QPW) * QS;
     const int qh = min(QS, F.nw - qr0), qw = min(QS, F.nh - qc0);
     const bool active = qh > 0 && qw > 0;
-    double* acc = s_acc[wave];
-    for (int k = lane; k < QS * QS; k += 64) acc[k] = 0.0;
+    double* acc = s_acc + wave * ACC;
+    for (int k = lane; k < QS * AS; k += 64) acc[k] = 0.0;
+    const char* kbase = (const char*)A.Kraw;
+    const unsigned lane24 = ((unsigned)lane << 24) | 0xffffffu;   // lane < lim <=> lane24 < magic_lim
+    const int acc_off = wave * (ACC * 8);                          // this wave's accumulator (bytes)
+    const unsigned spill = (unsigned)((WPT * ACC + lane) * 8);
 
     const int begin = A.tile_begin[tile];
     int count = A.tile_begin[tile + 1] - begin;
@@ -765,29 +806,32 @@ __global__ void __launch_bounds__(64 * QPW * QPW) splat_phase_kernel(FastArgs A)
             s_str[threadIdx.x] = make_int2(r.su, r.sv);
             s_flags[threadIdx.x] = r.flags;
             s_w[threadIdx.x] = r.w;
-            s_kb[threadIdx.x] = (const KT*)A.Kraw + (r.flags == 0 ? r.table_off : 0);
+            s_ko[threadIdx.x] = r.flags == 0 ? r.table_off : 0;
             if (r.flags & 2) atomicOr(A.status, 8);
         }
         __syncthreads();
         if (active) {
             // footprint rectangle of chunk emitter `lane` inside this quadrant:
-            // s_qi = {first pixel r0 * QS + c0, W | magic << 5, area, su | sv << 16},
-            // s_qk = table element of the rectangle's first pixel
-            int np = 0;
+            // origin (rr0, cc0) in the quadrant, width W, `area` pixels in np
+            // parts, kq = table element of its first pixel
+            int np = 0, W = 1, area = 0, rr0 = 0, cc0 = 0, su = 0, sv = 0;
+            long long kq = 0;
             bool irr = false;
             if (lane < nc && s_flags[lane] != 2) {
                 const int4 g = s_geo[lane];
                 const int r0 = max(g.x, qr0), r1 = min(g.x + g.z, qr0 + qh);
                 const int c0 = max(g.y, qc0), c1 = min(g.y + g.w, qc0 + qw);
                 if (r1 > r0 && c1 > c0) {
-                    const int W = c1 - c0, area = (r1 - r0) * W;
                     const int2 st = s_str[lane];
+                    W = c1 - c0;
+                    area = (r1 - r0) * W;
                     np = (area + 63) >> 6;
                     irr = s_flags[lane] == 1;
-                    s_qi[wave][lane] = make_int4((r0 - qr0) * QS + (c0 - qc0),
-                                                 W | (((65536 + W - 1) / W) << 5), area,
-                                                 (int)(((unsigned)st.x & 0xffffu) | ((unsigned)st.y << 16)));
-                    s_qk[wave][lane] = s_kb[lane] + (r0 - g.x) * st.x + (c0 - g.y) * st.y;
+                    rr0 = r0 - qr0;
+                    cc0 = c0 - qc0;
+                    su = st.x;
+                    sv = st.y;
+                    kq = s_ko[lane] + (long long)(r0 - g.x) * su + (long long)(c0 - g.y) * sv;
                 }
             }
             const bool any_irregular = __ballot(irr) != 0;
@@ -797,58 +841,110 @@ __global__ void __launch_bounds__(64 * QPW * QPW) splat_phase_kernel(FastArgs A)
                 int t = __shfl_up(incl, d);
                 if (lane >= d) incl += t;
             }
-            const int total = __shfl(incl, 63);
-            for (int p = 0; p < np; ++p) s_part[wave][incl - np + p] = (lane << 2) | p;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int first = incl - np;   // this emitter's parts: [first, incl) in list order
+            const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
 
             if (any_irregular) {
                 // rare (a float64 tie in a cell boundary): one part at a time, in
                 // order, flagged emitters integrated through the SAT
                 for (int b = 0; b < total; ++b) {
-                    const int ent = s_part[wave][b], c = ent >> 2;
-                    const int4 I = s_qi[wave][c];
-                    const int idx = (ent & 3) * 64 + lane;
-                    if (idx >= I.z) continue;
-                    const int W = I.y & 31;
-                    const int row = idx / W, col = idx - row * W;
-                    const int pq = I.x + row * QS + col;
+                    const int c = __popcll(__ballot(incl <= b));   // emitter lane owning part b
+                    const int p = b - __shfl(first, c);
+                    const int Wc = __shfl(W, c), ar = __shfl(area, c);
+                    const int rc = __shfl(rr0, c), cc = __shfl(cc0, c);
+                    const int suc = __shfl(su, c), svc = __shfl(sv, c);
+                    const long long kqc = __shfl(kq, c);
+                    const int idx = p * 64 + lane;
+                    if (idx >= ar) continue;
+                    const int row = idx / Wc, col = idx - row * Wc;
                     double val;
                     if (s_flags[c] == 1) {
-                        val = sat_pixel(A, F, s_e[c], s_w[c], qr0 + (pq / QS), qc0 + (pq % QS), 0.0);
+                        val = sat_pixel(A, F, s_e[c], s_w[c], qr0 + rc + row, qc0 + cc + col, 0.0);
                     } else {
-                        const int su = (short)(I.w & 0xffff), sv = I.w >> 16;
-                        val = (double)smx::gload(s_qk[wave][c] + row * su + col * sv) * s_w[c];
+                        val = (double)smx::gload((const KT*)A.Kraw + kqc + (long long)row * suc +
+                                                 (long long)col * svc) *
+                              s_w[c];
                     }
-                    lds_add(acc + pq, val);
+                    lds_add(acc + (rc + row) * AS + cc + col, val);
                 }
             } else {
-                for (int b = 0; b < total; b += NSLOT) {
-                    // branch-free part decoding (24-bit multiplies, masked
-                    // offsets), so the NSLOT table loads issue back to back
-                    KT kv[NSLOT];
-                    double wv[NSLOT];
-                    int pix[NSLOT];
-#pragma unroll
-                    for (int s = 0; s < NSLOT; ++s) {
-                        const int ent = s_part[wave][min(b + s, total - 1)], c = ent >> 2;
-                        const int4 I = s_qi[wave][c];
-                        const KT* kq = s_qk[wave][c];
-                        wv[s] = s_w[c];
-                        const int idx = (ent & 3) * 64 + lane;
-                        const bool ok = (b + s < total) & (idx < I.z);
-                        const int W = I.y & 31, magic = I.y >> 5;
-                        const int row = (int)(__umul24((unsigned)idx, (unsigned)magic) >> 16);
-                        const int col = idx - __mul24(row, W);
-                        const int su = (short)(I.w & 0xffff), sv = I.w >> 16;
-                        const int off = (__mul24(row, su) + __mul24(col, sv)) & -(int)ok;
-                        kv[s] = smx::gload(kq + off);
-                        pix[s] = ok ? I.x + row * QS + col : -1;
+                PartRec* pr = s_pr[wave];
+                double* pw = s_pw[wave];
+                const int magic = (65536 + W - 1) / W;
+                const double w = lane < nc ? s_w[lane] : 0.0;
+                const long long kb = kq * (long long)sizeof(KT);
+                const unsigned long long ka = (unsigned long long)(kbase + kb);
+                const int pix = acc_off + (rr0 * AS + cc0) * 8;
+                for (int rb = 0; rb < total; rb += PMAX) {
+                    // this round's parts [rb, rb + PMAX), then null records up to
+                    // a whole number of NSLOT groups (no lane valid, table offset 0)
+                    for (int p = 0; p < np; ++p) {
+                        const int pos = first + p - rb;
+                        if ((unsigned)pos >= (unsigned)PMAX) continue;
+                        const int lim = min(64, area - p * 64);
+                        PartRec& R = pr[pos];
+                        R.kq = O32 ? (int)(unsigned)kb : (int)(unsigned)ka;
+                        R.magic_lim = magic | (lim << 24);
+                        R.negw = -W;
+                        R.p64 = p * 64;
+                        R.su = su * (int)sizeof(KT);
+                        R.sv = sv * (int)sizeof(KT);
+                        R.pix = pix;
+                        R.kq_hi = O32 ? 0 : (int)(unsigned)(ka >> 32);
+                        pw[pos] = w;
                     }
+                    const int cnt = min(PMAX, total - rb);
+                    const int cntp = (cnt + NSLOT - 1) & ~(NSLOT - 1);
+                    if (lane >= cnt && lane < cntp) {
+                        PartRec& R = pr[lane];
+                        R.kq = O32 ? 0 : (int)(unsigned)(unsigned long long)kbase;
+                        R.magic_lim = 0;
+                        R.negw = 0;
+                        R.p64 = 0;
+                        R.su = 0;
+                        R.sv = 0;
+                        R.pix = 0;
+                        R.kq_hi = O32 ? 0 : (int)(unsigned)((unsigned long long)kbase >> 32);
+                        pw[lane] = 0.0;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (int b = 0; b < cntp; b += NSLOT) {
+                        KT kv[NSLOT];
+                        double wv[NSLOT];
+                        unsigned la[NSLOT];
 #pragma unroll
-                    for (int s = 0; s < NSLOT; ++s)
-                        if (pix[s] >= 0) lds_add(acc + pix[s], (double)kv[s] * wv[s]);
+                        for (int s = 0; s < NSLOT; ++s) {
+                            const PartRec& R = pr[b + s];
+                            const int4 h = *(const int4*)&R.kq;
+                            const int4 g = *(const int4*)&R.su;
+                            wv[s] = pw[b + s];
+                            const unsigned idx = (unsigned)lane + (unsigned)h.w;
+                            const bool ok = lane24 < (unsigned)h.y;
+                            const int row = (int)(__umul24(idx, (unsigned)h.y) >> 16);
+                            const int col = __mul24(row, h.z) + (int)idx;
+                            if (O32) {
+                                int off = __mul24(col, g.y) + (__mul24(row, g.x) + h.x);
+                                off = ok ? off : 0;
+                                kv[s] = smx::gload((const KT*)(kbase + (unsigned)off));
+                            } else {
+                                int off = __mul24(col, g.y) + __mul24(row, g.x);
+                                off = ok ? off : 0;
+                                const char* kp = (const char*)(((unsigned long long)(unsigned)g.w << 32) |
+                                                               (unsigned)h.x);
+                                kv[s] = smx::gload((const KT*)(kp + off));
+                            }
+                            const unsigned l = (unsigned)__mul24(row, AS * 8) + (unsigned)g.z + ((unsigned)col << 3);
+                            la[s] = ok ? l : spill;
+                        }
+#pragma unroll
+                        for (int s = 0; s < NSLOT; ++s)
+                            lds_add((double*)((char*)s_acc + la[s]), (double)kv[s] * wv[s]);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
         }
@@ -857,13 +953,14 @@ __global__ void __launch_bounds__(64 * QPW * QPW) splat_phase_kernel(FastArgs A)
     if (active) {
         for (int k = lane; k < QS * QS; k += 64) {
             const int r = k / QS, c = k % QS;
+            const double a = acc[r * AS + c];
             if (r >= qh || c >= qw) continue;
             const size_t o = (size_t)F.pix_off + (size_t)(qr0 + r) * F.nh + (qc0 + c);
             if (A.accumulate) {
-                if (count > 0) A.out[o] = A.out[o] + acc[k];
+                if (count > 0) A.out[o] = A.out[o] + a;
             } else {
                 // streaming store: keep the phase table, not the frame, in L2
-                __builtin_nontemporal_store(acc[k], A.out + o);
+                __builtin_nontemporal_store(a, A.out + o);
             }
         }
     }
@@ -1055,14 +1152,25 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
         B.accumulate = accumulate; B.out = out; B.status = status_dev;
         const dim3 grid((unsigned)n_tiles);
+        // table byte offsets fit 32 bits: parts address it from its base
+        const size_t slot_elems = k_format == 2 ? PF.elems() : smx_phase_table_elems(sat_period, m);
+        const double table_bytes = (double)n_slots * (double)slot_elems * (k_format != 0 ? 4.0 : 8.0);
+        // SMX_SPLAT_ADDR64=1 takes the 64-bit form for any table (tests of that form)
+        const char* a64 = getenv("SMX_SPLAT_ADDR64");
+        const bool o32 = table_bytes < 4294967296.0 && !(a64 && a64[0] == '1');
+        auto launch = [&](auto kt, auto o) {
+            using KT = decltype(kt);
+            constexpr bool O = decltype(o)::value;
+            if (lts == 4) hipLaunchKernelGGL((splat_phase_kernel<KT, 1, O>), grid, dim3(64), 0, stream, B);
+            else if (lts == 5) hipLaunchKernelGGL((splat_phase_kernel<KT, 2, O>), grid, dim3(256), 0, stream, B);
+            else hipLaunchKernelGGL((splat_phase_kernel<KT, 4, O>), grid, dim3(1024), 0, stream, B);
+        };
         if (k_format != 0) {
-            if (lts == 4) hipLaunchKernelGGL((splat_phase_kernel<float, 1>), grid, dim3(64), 0, stream, B);
-            else if (lts == 5) hipLaunchKernelGGL((splat_phase_kernel<float, 2>), grid, dim3(256), 0, stream, B);
-            else hipLaunchKernelGGL((splat_phase_kernel<float, 4>), grid, dim3(1024), 0, stream, B);
+            if (o32) launch(0.0f, std::true_type());
+            else launch(0.0f, std::false_type());
         } else {
-            if (lts == 4) hipLaunchKernelGGL((splat_phase_kernel<double, 1>), grid, dim3(64), 0, stream, B);
-            else if (lts == 5) hipLaunchKernelGGL((splat_phase_kernel<double, 2>), grid, dim3(256), 0, stream, B);
-            else hipLaunchKernelGGL((splat_phase_kernel<double, 4>), grid, dim3(1024), 0, stream, B);
+            if (o32) launch(0.0, std::true_type());
+            else launch(0.0, std::false_type());
         }
         SMX_CHECK_LAUNCH("splat_phase_kernel");
         return SMX_OK;

@@ -90,6 +90,67 @@ def test_tile_sides_give_identical_frames(gpu, precision):
         BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, tile_side=24)
 
 
+# EmitterRec as the phase-table path stores it at the start of the workspace
+_REC = numpy.dtype({'names': ['lo_u', 'lo_v', 'n_u', 'n_v', 'su', 'sv', 'flags', 'w', 'table_off'],
+                    'formats': ['<i4', '<i4', '<i2', '<i2', '<i2', '<i2', '<i4', '<f8', '<i8'],
+                    'offsets': [0, 4, 8, 10, 12, 14, 16, 24, 32], 'itemsize': 40})
+
+
+@pytest.mark.parametrize('precision,rtol', [('float64', 1e-11), ('folded', 1e-6)])
+@pytest.mark.parametrize('tile_side', [16, 32])
+def test_irregular_emitters_match_oracle(gpu, monkeypatch, precision, rtol, tile_side):
+    """Emitters on the 1 nm grid of the PSF table put pixel edges exactly on
+    table cells, where float64 rounding of a boundary quotient can leave the
+    regular phase grid: such emitters are flagged and integrated through the
+    SAT, one part at a time, in list order with the regular ones.  Half the
+    emitters here sit on the grid (nearly all flagged), half are random; the
+    frame equals the oracle, and the 64-bit table-address form of the tile
+    pass gives the same bits as the 32-bit one."""
+    from scopyon_amd.batched import BatchedSimulator
+    shape, k = (64, 64), 300
+    sim = BatchedSimulator(_cfg(shape), [shape], [k], dt=0.033, seed=9, tile_side=tile_side,
+                           table_precision=precision)
+    rng = numpy.random.RandomState(4)
+    pos = rng.uniform(-5e-6, 5e-6, size=(k, 2))
+    pos[::2] = rng.randint(-5000, 5000, size=(k // 2, 2)) * 1e-9
+    sim.u.copy_(torch.from_numpy(pos[:, 0].copy()))
+    sim.v.copy_(torch.from_numpy(pos[:, 1].copy()))
+    sim.form()
+    assert sim.status() & ~1 == 0
+    rec = numpy.frombuffer(sim.ws[:k * _REC.itemsize].cpu().numpy().tobytes(), dtype=_REC)
+    flagged = rec['flags'] == 1
+    assert flagged[::2].sum() > k // 4 and not flagged[1::2].any()
+    p = oracle_params({'detector': {'image_size': list(shape)}, 'magnification': 100.0})
+    parts = numpy.column_stack([numpy.zeros(k), pos[:, 0], pos[:, 1], numpy.arange(k), numpy.ones(k)])
+    ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, shape, numpy.zeros(3), 0.033, {}, None, None)
+    got = sim.frame(0, 'photons').cpu().numpy()
+    numpy.testing.assert_allclose(got, ref, rtol=rtol, atol=rtol * 1e-2 * ref.max())
+    first = sim.photons.cpu().numpy()
+    monkeypatch.setenv('SMX_SPLAT_ADDR64', '1')
+    sim.form()
+    assert numpy.array_equal(sim.photons.cpu().numpy().view(numpy.uint64), first.view(numpy.uint64))
+
+
+@pytest.mark.parametrize('precision', ['float64', 'float32', 'folded'])
+def test_table_address_forms_agree(gpu, monkeypatch, precision):
+    """The tile pass addresses the phase table with 32-bit byte offsets from
+    its base when the table spans less than 4 GiB, else with 64-bit addresses
+    per part; both forms give the same bits (tiles of 16, 32, 64)."""
+    from scopyon_amd.batched import BatchedSimulator
+    shapes = [(64, 64), (48, 80), (128, 128), (200, 72)]
+    counts = [50, 10, 900, 120]
+    for ts in (16, 32, 64):
+        monkeypatch.delenv('SMX_SPLAT_ADDR64', raising=False)
+        sim = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=21, table_precision=precision,
+                               tile_side=ts)
+        sim.form()
+        assert sim.status() & ~1 == 0
+        a = sim.photons.cpu().numpy()
+        monkeypatch.setenv('SMX_SPLAT_ADDR64', '1')
+        sim.form()
+        assert numpy.array_equal(sim.photons.cpu().numpy().view(numpy.uint64), a.view(numpy.uint64)), ts
+
+
 def test_graph_replay_equals_eager(gpu):
     from scopyon_amd.batched import BatchedSimulator
     shapes = [(64, 64), (32, 96)]
