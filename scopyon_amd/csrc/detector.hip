@@ -198,13 +198,15 @@ __device__ inline float log_factorial(float kf) {
 // Poisson for lam >= 12: Hormann's PTRS (transformed rejection with squeeze,
 // Insurance: Mathematics and Economics 12, 1993), two proposals per Philox
 // block.  log(lam) stays the accurate logf (it is multiplied by k in the
-// acceptance test); the other logarithms run on v_log_f32.
+// acceptance test); the other logarithms run on v_log_f32, square root and
+// reciprocals on v_sqrt_f32 / v_rcp_f32 (~1 ulp: the constants of the method
+// are themselves fitted to 3-4 digits).
 __device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
-    const float slam = sqrtf(lam), loglam = logf(lam);
+    const float slam = __builtin_amdgcn_sqrtf(lam), loglam = logf(lam);
     const float b = 0.931f + 2.53f * slam;
     const float aa = -0.059f + 0.02483f * b;
-    const float log_invalpha = ln_hw(1.1239f + 1.1328f / (b - 3.4f));
-    const float vr = 0.9277f - 3.6224f / (b - 2.0f);
+    const float log_invalpha = ln_hw(1.1239f + 1.1328f * __builtin_amdgcn_rcpf(b - 3.4f));
+    const float vr = 0.9277f - 3.6224f * __builtin_amdgcn_rcpf(b - 2.0f);
     for (uint32_t blk = 1; blk <= 256; ++blk) {
         const smx::u32x4 r = R.block(blk);
 #pragma unroll
@@ -215,7 +217,7 @@ __device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
             const float kf = floorf((2.0f * aa * __builtin_amdgcn_rcpf(us) + b) * U + lam + 0.43f);
             if (us >= 0.07f && V <= vr) return (int)kf;
             if (kf < 0.0f || (us < 0.013f && V > us)) continue;
-            if (ln_hw(V) + log_invalpha - ln_hw(aa / (us * us) + b) <=
+            if (ln_hw(V) + log_invalpha - ln_hw(aa * __builtin_amdgcn_rcpf(us * us) + b) <=
                 -lam + kf * loglam - log_factorial(kf))
                 return (int)kf;
         }
@@ -232,7 +234,7 @@ __device__ inline int poisson_ptrs(const PixelRng& R, float lam) {
 __device__ inline float gamma_mt(const PixelRng& R, int n, uint32_t& blk, bool first, float x0,
                                 float u0) {
     const float d = (float)n - 1.0f / 3.0f;
-    const float c = 1.0f / sqrtf(9.0f * d);
+    const float c = __builtin_amdgcn_rsqf(9.0f * d);   // v_rsq_f32, ~1 ulp
     for (int it = 0; it < 256; ++it) {
         float x, u;
         if (it == 0 && first) {
@@ -303,7 +305,7 @@ __device__ inline int queue_push(bool pred, int* counter) {
 // shortcut).
 __device__ inline float gamma_mt_from(const PixelRng& R, int n, uint32_t& blk, int it0) {
     const float d = (float)n - 1.0f / 3.0f;
-    const float c = 1.0f / sqrtf(9.0f * d);
+    const float c = __builtin_amdgcn_rsqf(9.0f * d);   // v_rsq_f32, ~1 ulp
     for (int it = it0; it < 256; ++it) {
         smx::u32x4 r = R.block(blk++);
         const float x = normal_f(r.x, r.y);
@@ -408,7 +410,7 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
             loc = s_q[q];
             const int k = s_k[loc];
             const float d = (float)k - 1.0f / 3.0f;
-            const float c = 1.0f / sqrtf(9.0f * d);
+            const float c = __builtin_amdgcn_rsqf(9.0f * d);   // v_rsq_f32, ~1 ulp
             const float x = s_x0[loc], u = s_u0[loc];
             float v = 1.0f + c * x;
             bool acc = false;
@@ -446,7 +448,7 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
             s = gamma_mt_from(R, k, blk, 1) * emgain;
         } else {
             const float d = (float)k - 1.0f / 3.0f;
-            const float c = 1.0f / sqrtf(9.0f * d);
+            const float c = __builtin_amdgcn_rsqf(9.0f * d);   // v_rsq_f32, ~1 ulp
             const float v = 1.0f + c * s_x0[loc];
             s = d * (v * v * v) * emgain;   // accepted, below 0.5
         }

@@ -147,6 +147,58 @@ def test_emccd_philox_readout_noise_is_gaussian(gpu):
     assert stats.kstest(x, 'norm').pvalue > 1e-3
 
 
+@pytest.mark.parametrize('E_val', [2.0, 30.0])
+def test_emccd_philox_distribution(gpu, E_val):
+    """Philox EMCCD register against the law it samples: k ~ Poisson(E), then
+    S = round(Gamma(k, g)) redrawn below 0.5 (S = 0 when k = 0), so
+    P(S <= s) = exp(-E) + sum_k pois(k) (F_k(s + 1/2) - F_k(1/2)) / (1 - F_k(1/2))
+    with F_k the Gamma(k, scale g) cdf.  Pearson chi-square over the atom at 0
+    (merged into the first bin when it expects < 20 pixels) and 30 bins of
+    equal probability (262,144 pixels, fixed seed) must not reject at 1e-4; a
+    1% error in the gain is rejected (checked with numpy's samplers)."""
+    from scipy import stats
+    from scopyon_amd import _lib, _device
+    p = oracle_params({'detector': {'image_size': [512, 512]}})
+    det = _det_params(p)
+    det.readout_noise = 0.0
+    det.background_switch = 0
+    det.gain, det.offset, det.fullwell, det.adc_max = 1.0, 0.0, 1e300, 1e300
+    n = 512 * 512
+    photons = torch.full((n,), E_val / p.QE, dtype=torch.float64, device=gpu)
+    adc = torch.empty(n, dtype=torch.float64, device=gpu)
+    E = torch.empty(n, dtype=torch.float64, device=gpu)
+    _lib.call('smx_emccd_philox', n, photons.data_ptr(), None, None, det, 2024, 11, 0, None,
+              E.data_ptr(), adc.data_ptr(), 0, _device.stream())
+    S = adc.cpu().numpy()
+    Ev = float(E.cpu().numpy()[0])
+    g = float(p.emgain)
+    assert (S == numpy.floor(S)).all() and (S >= 0).all()
+    ks = numpy.arange(1, int(Ev + 12 * numpy.sqrt(Ev) + 20))
+    wk = stats.poisson.pmf(ks, Ev)
+    s_max = int(g * (ks[-1] + 10 * numpy.sqrt(ks[-1])))
+    sv = numpy.arange(0, s_max + 1, dtype=float)
+    F = stats.gamma.cdf(sv[None, :] + 0.5, ks[:, None], scale=g)
+    F0 = stats.gamma.cdf(0.5, ks, scale=g)[:, None]
+    cdf = numpy.exp(-Ev) + (wk[:, None] * (F - F0) / (1.0 - F0)).sum(axis=0)   # P(S <= s)
+    p0 = numpy.exp(-Ev)
+    # the atom at 0 is a bin of its own when it expects >= 20 pixels
+    atom = p0 * n >= 20
+    lo = 0 if atom else -1
+    targets = p0 + (1.0 - p0) * numpy.arange(1, 30) / 30.0
+    uppers = numpy.unique(numpy.searchsorted(cdf, targets))        # bins (prev, upper]
+    edges = numpy.concatenate([[lo], uppers, [s_max + 10 ** 9]])
+    obs = [((S > a) & (S <= b)).sum() for a, b in zip(edges[:-1], edges[1:])]
+    probs = numpy.diff(numpy.concatenate([[p0 if atom else 0.0], cdf[uppers], [1.0]]))
+    if atom:
+        obs, probs = [(S == 0).sum()] + obs, numpy.concatenate([[p0], probs])
+    obs = numpy.array(obs)
+    exp = probs * n
+    assert obs.sum() == n and exp.min() > 20
+    chi2 = float(((obs - exp) ** 2 / exp).sum())
+    pval = stats.chi2.sf(chi2, len(exp) - 1)
+    assert pval > 1e-4, (E_val, chi2, len(exp), pval)
+
+
 @pytest.mark.parametrize('lam', [0.5, 5.0, 11.9, 12.0, 30.0, 150.0, 2000.0])
 def test_philox_poisson_sampler_distribution(gpu, lam):
     """The device Poisson sampler (inversion below 12, PTRS from 12 up, shared
