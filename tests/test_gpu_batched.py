@@ -131,6 +131,29 @@ def test_irregular_emitters_match_oracle(gpu, monkeypatch, precision, rtol, tile
     assert numpy.array_equal(sim.photons.cpu().numpy().view(numpy.uint64), first.view(numpy.uint64))
 
 
+def test_long_tile_lists_match_oracle(gpu):
+    """6,000 emitters on one 64 x 64 frame give tile lists longer than the
+    in-LDS sort holds (1,024 entries for 16 px tiles, 2,048 for 32 and 64 px):
+    those tiles are processed in list order, not emitter order (status bit 1),
+    and still equal the oracle to rounding (float64 tables)."""
+    from scopyon_amd.batched import BatchedSimulator
+    shape, k = (64, 64), 6000
+    p = oracle_params({'detector': {'image_size': list(shape)}, 'magnification': 100.0})
+    ref = None
+    for ts in (16, 32, 64):
+        sim = BatchedSimulator(_cfg(shape), [shape], [k], dt=0.033, seed=17, tile_side=ts,
+                               table_precision='float64')
+        sim.form()
+        assert sim.status() & 1 and sim.status() & ~1 == 0
+        if ref is None:
+            u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
+            parts = numpy.column_stack([numpy.zeros(k), u, v, numpy.arange(k), numpy.ones(k)])
+            ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, shape, numpy.zeros(3), 0.033, {},
+                                         None, None)
+        got = sim.frame(0, 'photons').cpu().numpy()
+        numpy.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * ref.max(), err_msg=str(ts))
+
+
 @pytest.mark.parametrize('precision', ['float64', 'float32', 'folded'])
 def test_table_address_forms_agree(gpu, monkeypatch, precision):
     """The tile pass addresses the phase table with 32-bit byte offsets from
