@@ -27,12 +27,24 @@ def depth_keys(depth, focal_depth, depth_cutoff):
 
 
 def sat_period(pixel_length):
-    """Layout period: the pixel length in table cells when it is an integer."""
+    """The pixel length in table cells when it is an integer (the phase-table
+    period), else 1."""
     ratio = pixel_length / RES
     p = int(round(ratio))
     if 1 <= p <= 4096 and abs(ratio - p) < 1e-6:
         return p
     return 1
+
+
+def layout_period(pixel_length):
+    """SAT layout period: the pixel length in cells rounded to an integer.
+    Pixel edges fall floor(ratio) or ceil(ratio) cells apart; with this period
+    the more frequent step keeps a corner's residue, so the corners of
+    neighbouring pixels mostly lie next to each other in memory (runs of ~2.6
+    corners per cache line at 66.4 cells instead of one)."""
+    ratio = pixel_length / RES
+    p = int(round(ratio))
+    return p if 2 <= p <= 4096 else 1
 
 
 class PSFTables:
@@ -49,7 +61,7 @@ class PSFTables:
         self.depth_cutoff = depth_cutoff
         self.nr = len(numpy.arange(0.0, radial_cutoff, RES, dtype=float))
         self.ntab = 2 * self.nr - 1
-        self.period = sat_period(pixel_length) if pixel_length else 1
+        self.period = layout_period(pixel_length) if pixel_length else 1
         self.sat_elems = int(_lib.lib().smx_psf_sat_elems(self.nr, self.period))
         # NA is 1.4 in the reference (_epifm.py:183); alpha, gamma as at :186-188
         k = 2.0 * numpy.pi / wavelength
@@ -65,7 +77,7 @@ class PSFTables:
         self.sat = None
         # phase tables (integer pixel/cell ratio only): footprint pixels per axis
         self.m = int(math.ceil(RES * (self.ntab - 1) / pixel_length)) + 2 if pixel_length else 0
-        self.use_phase = self.period > 1
+        self.use_phase = bool(pixel_length) and sat_period(pixel_length) > 1
         self.k_elems = (int(_lib.lib().smx_phase_table_elems(self.period, self.m))
                         if self.use_phase else 0)
         self.K = None

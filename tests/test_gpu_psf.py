@@ -35,11 +35,11 @@ def test_gaussian_radial_matches_oracle(gpu):
     numpy.testing.assert_allclose(got, ref, rtol=1e-14, atol=1e-300)
 
 
-@pytest.mark.parametrize('period,pl', [(1, 6.639004149377593e-08), (160, 1.6e-7)])
+@pytest.mark.parametrize('period,pl', [(66, 6.639004149377593e-08), (160, 1.6e-7)])
 def test_sat_pixel_sums_match_bruteforce(gpu, period, pl):
     from scopyon_amd import _lib
     t = _tables(gpu, pl=pl)
-    assert t.period == period
+    assert t.period == period and t.use_phase == (period == 160)
     t.ensure([0])
     r = O.radial_grid(1000e-9)
     prof = t.radial_last[0].cpu().numpy()
