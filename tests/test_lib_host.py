@@ -69,13 +69,36 @@ def test_np_remainder_matches_numpy():
         assert numpy.array_equal(got, ref)
 
 
-@pytest.mark.parametrize('side,period', [(2000, 1), (2000, 160), (2000, 66), (37, 5)])
+@pytest.mark.parametrize('side,period', [(2000, 1), (2000, 160), (2000, 66), (2000, 267), (37, 5)])
 def test_sat_layout_is_a_bijection(side, period):
     lib = _lib.lib()
     idx = numpy.array([[lib.smx_sat_index_host(side, period, a, b) for b in range(0, side, 7)]
                        for a in range(0, side, 7)])
     assert len(numpy.unique(idx)) == idx.size
     assert idx.max() < lib.smx_psf_sat_elems(side // 2, period) or side % 2
+
+
+@pytest.mark.parametrize('pl,layout,phase', [
+    (160e-9, 160, 160),                  # 16 um / 100x: integer, phase tables
+    (16e-6 / 241.0, 66, 1),              # 66.39 cells: layout period only
+    (16e-6 / 60.0, 267, 1),              # 266.67 cells rounds up
+    (1.2e-9, 1, 1),                      # below two cells: row-major
+])
+def test_sat_periods(pl, layout, phase):
+    """The SAT layout period is the pixel length in cells rounded (corners of
+    neighbouring pixels, floor or ceil of it apart, then mostly share a
+    residue); phase tables need an exact integer ratio."""
+    from scopyon_amd import _psf
+    assert _psf.layout_period(pl) == layout
+    assert _psf.sat_period(pl) == phase
+    if layout > 1:
+        # the more frequent of the two edge spacings keeps the residue
+        ratio = pl / 1e-9
+        edges = numpy.ceil((numpy.arange(40) * ratio + 0.3) - 1e-9).astype(int)
+        steps = numpy.diff(edges)
+        assert set(steps) <= {int(numpy.floor(ratio)), int(numpy.ceil(ratio))}
+        kept = (numpy.diff(edges % layout) == 0).mean()
+        assert kept >= 0.5
 
 
 def test_bad_arguments_raise():
