@@ -737,10 +737,11 @@ struct __align__(16) PartRec {
     int kq_hi;       // high half of the byte address (64-bit form)
 };
 constexpr int PMAX = 64;          // parts per wave per round (one lane writes each padding record)
-// waves per SIMD the register allocation leaves room for (5 at 256 threads:
-// the LDS of 5 blocks fits a CU)
+// waves per SIMD the register allocation leaves room for: 6 (<= 80 VGPRs; a
+// 32-pixel tile's block takes 22.5 KB of LDS, so 7 fit a CU; 7 waves would
+// spill)
 #ifndef SMX_SPLAT_WPE
-#define SMX_SPLAT_WPE 5
+#define SMX_SPLAT_WPE 6
 #endif
 
 template <typename KT, int QPW, bool O32>
@@ -749,9 +750,11 @@ splat_phase_kernel(FastArgs A) {
     constexpr int WPT = QPW * QPW, TSZ = QS * QPW, PSORT_CAP = psort_cap<QPW>();
     constexpr int ACC = QS * AS;
     // accumulators, then one spill slot per lane shared by the waves (lanes
-    // outside a part add there: no branch per part, the slots are never read)
-    __shared__ double s_acc[WPT * ACC + 64];
-    __shared__ int s_sort[PSORT_CAP];
+    // outside a part add there: no branch per part, the slots are never read);
+    // before the accumulators are cleared the same space sorts the tile list
+    constexpr int ACCN = WPT * ACC + 64 > PSORT_CAP / 2 ? WPT * ACC + 64 : PSORT_CAP / 2;
+    __shared__ double s_acc[ACCN];
+    int* const s_sort = (int*)s_acc;
     __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (frame pixels), n_u, n_v
     __shared__ int2 s_str[FCHUNK];            // table strides per pixel along u, v
     __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
@@ -771,7 +774,6 @@ splat_phase_kernel(FastArgs A) {
     const int qh = min(QS, F.nw - qr0), qw = min(QS, F.nh - qc0);
     const bool active = qh > 0 && qw > 0;
     double* acc = s_acc + wave * ACC;
-    for (int k = lane; k < QS * AS; k += 64) acc[k] = 0.0;
     const char* kbase = (const char*)A.Kraw;
     const unsigned lane24 = ((unsigned)lane << 24) | 0xffffffu;   // lane < lim <=> lane24 < magic_lim
     const int acc_off = wave * (ACC * 8);                          // this wave's accumulator (bytes)
@@ -780,13 +782,13 @@ splat_phase_kernel(FastArgs A) {
     const int begin = A.tile_begin[tile];
     int count = A.tile_begin[tile + 1] - begin;
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
-    const int* list = A.list + begin;
+    int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && count <= WAVE_SORT_MAX) {
+    if (count >= 2 && count <= WAVE_SORT_MAX) {
         // list of up to 1024 entries: one wave sorts it in registers
         if (wave == 0) wave_sort_list(list, count, lane, s_sort);
         __syncthreads();
-    } else if (count >= 1 && sorted) {
+    } else if (count >= 2 && sorted) {
         int np2 = 1;
         while (np2 < count) np2 <<= 1;
         for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
@@ -795,11 +797,18 @@ splat_phase_kernel(FastArgs A) {
     } else if (!sorted && threadIdx.x == 0) {
         atomicOr(A.status, 1);
     }
+    if (count >= 2 && sorted) {
+        // sorted list back in place (read per chunk below), then the space
+        // becomes the accumulators
+        for (int k = threadIdx.x; k < count; k += blockDim.x) list[k] = s_sort[k];
+        __syncthreads();
+    }
+    for (int k = lane; k < QS * AS; k += 64) acc[k] = 0.0;
 
     for (int base = 0; base < count; base += FCHUNK) {
         const int nc = min(FCHUNK, count - base);
         if (threadIdx.x < nc) {
-            int e = sorted ? s_sort[base + threadIdx.x] : smx::gload(list + base + threadIdx.x);
+            int e = smx::gload(list + base + threadIdx.x);
             EmitterRec r = A.rec[e];
             s_e[threadIdx.x] = e;
             s_geo[threadIdx.x] = make_int4(r.lo_u, r.lo_v, r.n_u, r.n_v);
