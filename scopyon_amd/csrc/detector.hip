@@ -329,21 +329,24 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
     __shared__ float s_lam[EQ_PIX];      // expectation (PTRS items)
     __shared__ float s_x0[EQ_PIX];       // spare normal of block 0
     __shared__ float s_u0[EQ_PIX];       // spare uniform of block 0
-    __shared__ float s_S[EQ_PIX];        // EM-register output
+    __shared__ float s_S[EQ_PIX];        // inversion uniform (phases 1-2), EM-register output
     __shared__ short s_q[EQ_PIX];        // PTRS queue, then the Gamma queue
-    __shared__ short s_r[EQ_PIX];        // Gamma retry queue (first draw rejected: flag 1 << 14)
-    __shared__ int s_n[3];               // PTRS, Gamma and retry queue lengths
+    __shared__ short s_r[EQ_PIX];        // inversion queue (phases 1-2), then the Gamma
+                                         // retry queue (first draw rejected: flag 1 << 14)
+    __shared__ int s_n[4];               // PTRS, Gamma, retry and inversion queue lengths
 
     const int t = threadIdx.x;
     const long long base = (long long)blockIdx.x * EQ_PIX;
-    if (t < 3) s_n[t] = 0;
+    if (t < 4) s_n[t] = 0;
     if (frame_dev) frame += *frame_dev;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const float emgain = (float)P.emgain;
     __syncthreads();
 
-    // phase 1: block 0 of every pixel (readout normal, spare normal, Poisson
-    // inversion below 12, spare uniform); PTRS pixels queued
+    // phase 1: block 0 of every pixel (readout normal, spare normal, spare
+    // uniform, Poisson uniform); k = 0 where the inversion's first term
+    // decides, the other pixels below 12 queued for the inversion and those
+    // from 12 up for PTRS (waves then drain pixels of similar work)
     float noise[EQ_PPT];
 #pragma unroll
     for (int j = 0; j < EQ_PPT; ++j) {
@@ -351,9 +354,8 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
         const long long i = base + loc;
         const bool in = i < n;
         noise[j] = 0.0f;
-        int k = 0;
         float E = 0.0f, x0 = 0.0f, u0 = 1.0f;
-        bool ptrs = false;
+        bool ptrs = false, inv = false;
         if (in) {
             float ph = (float)photons[i];
             if (P.background_switch) ph += (float)P.background;
@@ -365,26 +367,39 @@ __global__ void __launch_bounds__(EQ_THREADS) emccd_queue_kernel(
             x0 = nn.y;
             u0 = u01f_open0(r0.w);
             if (E > 0.0f) {
-                if (E < 12.0f) k = poisson_inversion(E, u01f(r0.z));
-                else ptrs = true;
+                if (E < 12.0f) {
+                    const float u = u01f(r0.z);
+                    // the inversion's first test (k = 0), as in poisson_inversion
+                    inv = u > __builtin_amdgcn_exp2f(-1.4426950408889634f * E);
+                    if (inv) s_S[loc] = u;
+                } else {
+                    ptrs = true;
+                }
             }
             if (E_out) E_out[i] = (double)E;
         }
-        s_k[loc] = k;
+        s_k[loc] = 0;   // queued pixels get theirs in phase 2
         s_lam[loc] = E;
         s_x0[loc] = x0;
         s_u0[loc] = u0;
         const int slot = queue_push(ptrs, &s_n[0]);
         if (ptrs) s_q[slot] = (short)loc;
+        const int islot = queue_push(inv, &s_n[3]);
+        if (inv) s_r[islot] = (short)loc;
     }
     __syncthreads();
 
-    // phase 2: PTRS (lam >= 12), one queued pixel per lane
-    const int np = s_n[0];
-    for (int q = t; q < np; q += EQ_THREADS) {
-        const int loc = s_q[q];
-        const PixelRng R{pix_base + (uint32_t)(base + loc), frame, k0, k1};
-        s_k[loc] = poisson_ptrs(R, s_lam[loc]);
+    // phase 2: PTRS (lam >= 12), then the inversions, one queued pixel per lane
+    const int np = s_n[0], ni = s_n[3];
+    for (int q = t; q < np + ni; q += EQ_THREADS) {
+        if (q < np) {
+            const int loc = s_q[q];
+            const PixelRng R{pix_base + (uint32_t)(base + loc), frame, k0, k1};
+            s_k[loc] = poisson_ptrs(R, s_lam[loc]);
+        } else {
+            const int loc = s_r[q - np];
+            s_k[loc] = poisson_inversion(s_lam[loc], s_S[loc]);
+        }
     }
     __syncthreads();
 
