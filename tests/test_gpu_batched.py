@@ -248,3 +248,33 @@ def test_batched_ccd_cmos_steps(gpu, kind):
     adc = sim.adc.cpu().numpy()
     assert numpy.isfinite(adc).all() and (adc >= 0).all()
     assert adc.std() > 0
+
+
+def test_empty_fovs(gpu):
+    """FOVs without emitters (and a batch without any) form frames of zeros,
+    detect them (readout noise, ADC offset) and replay from a HIP graph; the
+    FOV between them equals the oracle."""
+    from scopyon_amd.batched import BatchedSimulator
+    shapes, counts = [(64, 64), (48, 80), (32, 32)], [0, 25, 0]
+    sim = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=41, tile_side=32,
+                           table_precision='float64')
+    sim.form()
+    assert sim.status() & ~1 == 0
+    assert not sim.frame(0, 'photons').cpu().numpy().any()
+    assert not sim.frame(2, 'photons').cpu().numpy().any()
+    u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
+    p = oracle_params({'detector': {'image_size': [48, 80]}, 'magnification': 100.0})
+    parts = numpy.column_stack([numpy.zeros(25), u, v, numpy.arange(25), numpy.ones(25)])
+    ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, (48, 80), numpy.zeros(3), 0.033, {},
+                                 None, None)
+    numpy.testing.assert_allclose(sim.frame(1, 'photons').cpu().numpy(), ref, rtol=1e-11,
+                                  atol=1e-13 * ref.max())
+    sim.capture()
+    sim.replay(2)
+    adc = sim.adc.cpu().numpy()
+    assert numpy.isfinite(adc).all() and (adc >= 0).all()
+    none = BatchedSimulator(_cfg(), [(64, 64), (32, 48)], [0, 0], dt=0.033, seed=42)
+    assert none.n == 0
+    none.capture()
+    none.replay(2)
+    assert not none.photons.cpu().numpy().any() and none.status() & ~1 == 0
