@@ -174,7 +174,7 @@ constexpr int BIN_THREADS = 1024;
 constexpr int BIN_WIN = 4096;
 
 __device__ inline void bin_hist_clear(int* s_hist) {
-    for (int k = threadIdx.x; k < BIN_WIN; k += BIN_THREADS) s_hist[k] = 0;
+    for (int k = threadIdx.x; k < BIN_WIN; k += blockDim.x) s_hist[k] = 0;
 }
 // count one entry of `tile`
 __device__ inline void bin_count(int* s_hist, int wbase, int tile, int* __restrict__ tile_count,
@@ -186,7 +186,7 @@ __device__ inline void bin_count(int* s_hist, int wbase, int tile, int* __restri
 // after a barrier: add the block's window counts to the global counters
 __device__ inline void bin_count_flush(const int* s_hist, int wbase, int* __restrict__ tile_count,
                                        int cs) {
-    for (int k = threadIdx.x; k < BIN_WIN; k += BIN_THREADS) {
+    for (int k = threadIdx.x; k < BIN_WIN; k += blockDim.x) {
         const int c = s_hist[k];
         if (c) atomicAdd(&tile_count[(size_t)(wbase + k) * cs], c);
     }
@@ -195,7 +195,7 @@ __device__ inline void bin_count_flush(const int* s_hist, int wbase, int* __rest
 // run down to 0 -- the block's entries of tile wbase + k take [s_hist[k],
 // s_hist[k] + count) on return
 __device__ inline void bin_claim(int* s_hist, int wbase, int* __restrict__ tile_left, int cs) {
-    for (int k = threadIdx.x; k < BIN_WIN; k += BIN_THREADS) {
+    for (int k = threadIdx.x; k < BIN_WIN; k += blockDim.x) {
         const int c = s_hist[k];
         if (c) s_hist[k] = atomicSub(&tile_left[(size_t)(wbase + k) * cs], c) - c;
     }
