@@ -63,11 +63,17 @@ class BatchedSimulator:
         # PSF tables: depth 0 only for 2D batches, every key for 3D batches
         self.tables = PSFTables(c.fluorophore_type, c.psf_wavelength, c.radial_cutoff,
                                 c.depth_cutoff, c.psf_radial_width, pl, self.device)
+        if boundary not in ('none', 'periodic', 'reflect'):
+            raise ValueError("boundary must be 'none', 'periodic' or 'reflect'")
         if depth_range is None:
             self.tables.ensure([0])
+        elif boundary == 'none':
+            # nothing keeps an emitter inside the depth range: every key up to
+            # the cutoff, so no emitter can meet an unbuilt table
+            self.tables.prebuild_all()
         else:
-            # every 1 nm key an emitter inside the depth range (walls keep it
-            # there) can reach, plus the cutoff key
+            # every 1 nm key an emitter inside the depth range (walls or the
+            # periodic wrap keep it there) can reach, plus the cutoff key
             dmax = max(abs(depth_range[0]), abs(depth_range[1]))
             self.tables.prebuild_range(dmax)
         self.slot_map = self.tables.slot_map()
@@ -310,7 +316,13 @@ class BatchedSimulator:
         return buf[a:b].view(g.nw, g.nh)
 
     def status(self):
+        """Status bits of the splats since construction: raises on a tile
+        list overflow (bit 2) or an emitter that met an unbuilt PSF table
+        (bit 8, its light would be missing); returns the word otherwise (bit
+        1: a tile list too long to sort, summed out of emitter order)."""
         s = int(self.splatter.status.item())
         if s & 2:
             raise RuntimeError('splat tile list capacity exceeded')
+        if s & 8:
+            raise RuntimeError('a PSF table needed by an emitter was not built')
         return s

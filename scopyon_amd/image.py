@@ -1,9 +1,20 @@
-"""Image container returned by ``form_image`` (src/scopyon/image.py:12-238).
+"""``Image``: the camera frame ``form_image`` returns (ADC counts), and
+``Video``: a stack of such frames written as one animated file.
 
-Keeps the reference's surface for what the hot path returns: ``as_array``,
-``as_8bit`` (bytescale), ``save`` (.npy / .csv / image formats through PIL),
-``load``, ``RGB`` and the shape/dtype properties.  ``Video.save`` writes an
-animation through matplotlib when it is installed.
+Public surface kept from the reference (src/scopyon/image.py:12-238) because
+callers use it: ``Image(data)``, ``as_array()``, ``shape`` / ``dtype`` /
+``ndim`` / ``size``, ``as_8bit(cmin, cmax, low, high)``, ``save(filename)``,
+``Image.load(filename)``, ``Image.RGB(red, green, blue)`` and
+``Video.save(filename, images, ...)``.
+
+The 8-bit conversion is the linear ``bytescale`` of image.py:99-123: values
+are mapped from [cmin, cmax] (default: the channel's min and max) onto
+[low, high] (default 0 and 255), clipped, and rounded half up.  Like the
+reference (whose uint8 short-cut at image.py:137 compares a dtype object with
+``is`` and never fires), uint8 data is rescaled too.  File formats follow
+image.py:152-175: ``.npy`` and ``.csv`` hold the raw values, any other suffix
+is an 8-bit picture written by Pillow.  The plotting helpers of the reference
+(plotly/matplotlib figures, spot overlays) are not part of this package.
 """
 import os
 import pathlib
@@ -13,141 +24,135 @@ import numpy
 __all__ = ["Image", "Video"]
 
 
+def _path(filename):
+    if not isinstance(filename, (str, pathlib.PurePath)):
+        raise TypeError('filename must be a str or a path, not {}'.format(type(filename).__name__))
+    name = os.fspath(filename)
+    return name, os.path.splitext(name)[1].lower()
+
+
+def _scale_channel(values, cmin, cmax, low, high):
+    """One channel onto [low, high] as uint8 (image.py:99-123 arithmetic)."""
+    lo_val = values.min() if cmin is None else cmin
+    hi_val = values.max() if cmax is None else cmax
+    low = 0 if low is None else low
+    high = 255 if high is None else high
+    span = hi_val - lo_val
+    if span == 0.0:
+        return numpy.full(values.shape, low, dtype=numpy.uint8)
+    k = float(high - low) / span
+    mapped = numpy.clip((values - lo_val) * k + low, low, high)
+    return (mapped + 0.5).astype(numpy.uint8)
+
+
+def _to_8bit(data, cmin, cmax, low, high):
+    if data.ndim == 2:
+        return _scale_channel(data, cmin, cmax, low, high)
+    return numpy.stack([_scale_channel(data[:, :, c], cmin, cmax, low, high)
+                        for c in range(data.shape[2])], axis=2)
+
+
 class Image(object):
-
-    @staticmethod
-    def load(file):
-        assert isinstance(file, (str, pathlib.PurePath))
-        filename = str(file)
-        ext = os.path.splitext(filename)[1].lower()
-        if ext == '.npy':
-            return Image(numpy.load(filename))
-        if ext == '.csv':
-            return Image(numpy.loadtxt(filename))
-        import PIL.Image
-        return Image(numpy.asarray(PIL.Image.open(filename)))
-
-    @staticmethod
-    def RGB(red=None, green=None, blue=None):
-        shape, dtype = None, None
-        for ch in (red, green, blue):
-            if ch is None:
-                continue
-            if shape is None:
-                shape, dtype = ch.shape, ch.dtype
-            else:
-                assert shape == ch.shape and dtype == ch.dtype
-        assert shape is not None
-        rgb = numpy.zeros((shape[0], shape[1], 3), dtype=dtype)
-        for i, ch in enumerate((red, green, blue)):
-            if isinstance(ch, Image):
-                rgb[:, :, i] = ch.as_array()
-            elif isinstance(ch, numpy.ndarray):
-                rgb[:, :, i] = ch
-            else:
-                assert ch is None
-        return Image(rgb)
+    """A 2D frame (H x W) or an RGB frame (H x W x 3)."""
 
     def __init__(self, data):
-        assert data.ndim == 2 or (data.ndim == 3 and data.shape[2] == 3)
-        self.__data = data
+        data = numpy.asarray(data)
+        if not (data.ndim == 2 or (data.ndim == 3 and data.shape[2] == 3)):
+            raise ValueError('an Image holds a 2D array or an RGB array (H, W, 3); got shape {}'
+                             .format(data.shape))
+        self._data = data
 
+    # ---- data access ----
     def as_array(self):
-        return self.__data
-
-    @property
-    def dtype(self):
-        return self.__data.dtype
-
-    @property
-    def ndim(self):
-        return self.__data.ndim
-
-    @property
-    def size(self):
-        return self.__data.size
+        return self._data
 
     @property
     def shape(self):
-        return self.__data.shape
+        return self._data.shape
 
-    @staticmethod
-    def _bytescale(data, cmin=None, cmax=None, low=None, high=None):
-        """image.py:99-123."""
-        cmin = data.min() if cmin is None else cmin
-        cmax = data.max() if cmax is None else cmax
-        low = 0 if low is None else low
-        high = 255 if high is None else high
-        cscale = cmax - cmin
-        if cscale == 0.0:
-            return numpy.ones(data.shape, dtype=numpy.uint8) * low
-        scale = float(high - low) / cscale
-        byte = (data - cmin) * scale + low
-        return (byte.clip(low, high) + 0.5).astype(numpy.uint8)
+    @property
+    def dtype(self):
+        return self._data.dtype
 
+    @property
+    def ndim(self):
+        return self._data.ndim
+
+    @property
+    def size(self):
+        return self._data.size
+
+    # ---- conversions ----
     def as_8bit(self, cmin=None, cmax=None, low=None, high=None):
-        if self.dtype == numpy.uint8:
-            data = self.__data.copy()
-        elif self.ndim == 2:
-            data = self._bytescale(self.__data, cmin, cmax, low, high)
-        else:
-            data = numpy.zeros(self.__data.shape, dtype=numpy.uint8)
-            for i in range(3):
-                data[:, :, i] = self._bytescale(self.__data[:, :, i], cmin, cmax, low, high)
-        return Image(data)
-
-    def save(self, filename, **kwargs):
-        assert isinstance(filename, (str, pathlib.PurePath))
-        filename = str(filename)
-        ext = os.path.splitext(filename)[1].lower()
-        if ext == '.npy':
-            assert not kwargs
-            numpy.save(filename, self.__data)
-        elif ext == '.csv':
-            assert not kwargs
-            numpy.savetxt(filename, self.__data)
-        else:
-            self.savefig(filename, self.as_8bit().as_array(), **kwargs)
+        """A new Image of uint8 values: each channel mapped linearly from
+        [cmin, cmax] (default: that channel's range) onto [low, high]
+        (default [0, 255])."""
+        return Image(_to_8bit(self._data, cmin, cmax, low, high))
 
     @staticmethod
-    def savefig(filename, img, shapes=None):
+    def RGB(red=None, green=None, blue=None):
+        """An RGB Image from up to three channels (2D arrays or Images of one
+        shape and dtype); a missing channel is zero."""
+        chans = [c.as_array() if isinstance(c, Image) else c for c in (red, green, blue)]
+        given = [c for c in chans if c is not None]
+        if not given:
+            raise ValueError('RGB() needs at least one channel')
+        for c in given:
+            if not isinstance(c, numpy.ndarray):
+                raise TypeError('a channel is a numpy array or an Image')
+            if c.shape != given[0].shape or c.dtype != given[0].dtype:
+                raise ValueError('RGB channels differ in shape or dtype')
+        out = numpy.zeros(given[0].shape[:2] + (3,), dtype=given[0].dtype)
+        for k, c in enumerate(chans):
+            if c is not None:
+                out[:, :, k] = c
+        return Image(out)
+
+    # ---- files ----
+    def save(self, filename, **kwargs):
+        """Write the frame: ``.npy`` (numpy binary) and ``.csv`` (text) keep
+        the values; other suffixes (``.png``, ``.tif``, ...) store
+        ``as_8bit()`` through Pillow."""
+        name, ext = _path(filename)
+        if kwargs:
+            raise TypeError('Image.save() takes no options here (got {})'.format(sorted(kwargs)))
+        if ext == '.npy':
+            numpy.save(name, self._data)
+        elif ext == '.csv':
+            numpy.savetxt(name, self._data)
+        else:
+            import PIL.Image
+            PIL.Image.fromarray(self.as_8bit().as_array()).save(name)
+
+    @staticmethod
+    def load(filename):
+        """Read a frame written by ``save`` (or any picture Pillow reads)."""
+        name, ext = _path(filename)
+        if ext == '.npy':
+            return Image(numpy.load(name))
+        if ext == '.csv':
+            return Image(numpy.loadtxt(name))
         import PIL.Image
-        from PIL import ImageDraw
-        pic = PIL.Image.fromarray(img)
-        if shapes is not None:
-            pic = pic.convert('RGB')
-            d = ImageDraw.Draw(pic)
-            for s in shapes:
-                x, y = s['y'], s['x']
-                d.rectangle([(x - s['sigma'], y - s['sigma']), (x + s['sigma'], y + s['sigma'])],
-                            outline=s['color'], width=1)
-        pic.save(filename)
-
-    def show(self, **kwargs):
-        import matplotlib.pyplot as plt
-        plt.imshow(self.as_8bit().as_array(), cmap='gray', interpolation='none', **kwargs)
-        plt.show()
+        with PIL.Image.open(name) as pic:
+            return Image(numpy.asarray(pic))
 
 
-class Video:
+class Video(object):
+    """Frames written as one animated picture (GIF, multi-page TIFF, ...)."""
 
     @staticmethod
     def save(filename, imgs, interval=100, dpi=None, cmin=None, cmax=None, low=None, high=None):
-        import matplotlib.pyplot as plt
-        from matplotlib.animation import FuncAnimation
-        if cmin is None:
-            cmin = min(img.as_array().min() for img in imgs)
-        if cmax is None:
-            cmax = max(img.as_array().max() for img in imgs)
-        plt.ioff()
-        fig, ax = plt.subplots(1, figsize=(1, 1))
-        fig.subplots_adjust(0, 0, 1, 1)
-        ax.axis('off')
-        art = ax.imshow(imgs[0].as_8bit(cmin, cmax, low, high).as_array(), cmap='gray')
-
-        def animate(k):
-            art.set_array(imgs[k].as_8bit(cmin, cmax, low, high).as_array())
-            return art
-        anim = FuncAnimation(fig, animate, numpy.arange(len(imgs)), interval=interval)
-        anim.save(filename, dpi=dpi or max(imgs[0].shape))
-        plt.close(fig)
+        """Write `imgs` (Images or arrays) with one 8-bit scale shared by all
+        frames (default: the global min and max), `interval` ms per frame.
+        `dpi` is accepted for call compatibility; frames keep their pixel
+        size."""
+        del dpi
+        name, _ = _path(filename)
+        arrays = [i.as_array() if isinstance(i, Image) else numpy.asarray(i) for i in imgs]
+        if not arrays:
+            raise ValueError('Video.save() needs at least one frame')
+        lo = min(a.min() for a in arrays) if cmin is None else cmin
+        hi = max(a.max() for a in arrays) if cmax is None else cmax
+        import PIL.Image
+        pics = [PIL.Image.fromarray(_to_8bit(a, lo, hi, low, high)) for a in arrays]
+        pics[0].save(name, save_all=True, append_images=pics[1:], duration=int(interval), loop=0)
