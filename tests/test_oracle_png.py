@@ -5,17 +5,20 @@
 100 emitters uniform over the 512 x 512 frame, default configuration, 33 ms
 exposure, ``Image.as_8bit`` (image.py:99-123).
 
-What this pins: the random-draw order of one frame (emitter uniforms, then the
-readout-noise vector, then one ``choice`` uniform per pixel,
-_epifm.py:355-416, 1430-1470) and the background -> QE -> EMCCD -> ADC chain.
-Pixels that only see the background are compared: the PNG was produced by an
-older revision of the PSF code, so pixels under emitters differ by a few per
-cent in expectation; their magnitude is pinned by the closed-form and
-brute-force PSF/overlay tests instead.
+What this pins:
+* background-only pixels: the random-draw order of one frame (emitter
+  uniforms, then the readout-noise vector, then one ``choice`` uniform per
+  pixel, _epifm.py:355-416, 1430-1470) and the background -> QE -> EMCCD ->
+  ADC chain, pixel for pixel;
+* pixels under emitters: the emitter brightness.  Given the reference's own
+  noise and uniforms, each bright pixel's grey level fixes the expectation
+  that produced it to about 1.4% (one grey level), and the median over 400
+  pixels to ~0.15%: the oracle's weight chain agrees with the image to
+  better than 0.5%, and a weight off by 1% does not.
 
-The 8-bit rescale's upper bound (the brightest pixel) is therefore not
-reproducible; the slope of the linear rescale is fitted on the background
-pixels, and the offset is the frame minimum as in ``as_8bit``.
+The 8-bit rescale's upper bound (the brightest pixel) depends on emitter
+brightness, so the slope of the linear rescale is fitted on the background
+pixels and the offset is the frame minimum, as in ``as_8bit``.
 """
 import os
 
@@ -138,3 +141,57 @@ def test_twocolor_png_background_pixels():
         assert n_bg > 100_000, (ch, n_bg)
         assert exact > 0.99, (ch, exact)
         assert worst <= 1, (ch, worst)
+
+
+def _emitter_scale(p, seen, adc, png, E):
+    """Per-pixel ratio E_png / E for the 400 brightest pixels under emitters.
+
+    The readout noise and the per-pixel ``choice`` uniforms are the
+    reference's draws (same RandomState stream).  The 8-bit slope is fitted
+    on the background pixels, which do not depend on emitter brightness; a
+    pixel's grey level then gives its ADC value to half a level, hence its
+    EM-register output S = (ADC - offset) * gain - noise, and E_png is the
+    expectation at which the EMCCD quantile function at that pixel's uniform
+    returns S (linear between the quantiles at 0.97 E, E and 1.03 E)."""
+    rng = numpy.random.RandomState()
+    rng.set_state(seen['state'])
+    noise = O.emccd_noise(E.shape, p.readout_noise, rng)
+    u = rng.random_sample(E.size).reshape(E.shape)     # every pixel has E > 0
+    E_bg = p.QE * (seen['camera'][:, :, 0] + p.background_mean)
+    bg = E_bg == E_bg.min()
+    x = adc[bg] - adc.min()
+    slope = (x * png[bg]).sum() / (x * x).sum()
+    cand = numpy.argwhere((E_bg > 5.0) & (png < 255))
+    top = cand[numpy.argsort(-E_bg[tuple(cand.T)], kind='stable')[:400]]
+    gain = numpy.broadcast_to(numpy.asarray(p.adc_gain, dtype=float).reshape(-1), (E.size,))
+    offset = numpy.broadcast_to(numpy.asarray(p.adc_offset, dtype=float).reshape(-1), (E.size,))
+    gain, offset = gain.reshape(E.shape), offset.reshape(E.shape)
+    ratios = []
+    d = 0.03
+    for i, j in top:
+        S_png = ((png[i, j] / slope + adc.min()) - offset[i, j]) * gain[i, j] - noise[i, j]
+        e = E[i, j]
+        q = [O.emccd_index(e * (1 + k * d), p.emgain, u[i, j]) for k in (-1, 0, 1)]
+        if S_png >= q[1]:
+            f = 1 + d * (S_png - q[1]) / max(q[2] - q[1], 1.0)
+        else:
+            f = 1 - d * (q[1] - S_png) / max(q[1] - q[0], 1.0)
+        ratios.append(f)
+    return numpy.array(ratios)
+
+
+def test_tirf_png_emitter_brightness(frame, png):
+    """Pixels under emitters: the oracle's emitter weight chain (snells_law x
+    get_emit_photons x spectral sum x psf_normalization / 4 pi,
+    _epifm.py:1280-1315, 1343-1428) reproduces the brightness the reference
+    rendered into examples/tirf_000.png: the median of E_png / E over the 400
+    brightest emitter pixels is 1 to within 0.5%; a weight 1% too large is
+    outside that band."""
+    p, cam, seen = frame
+    E = p.QE * (seen['camera'][:, :, 0] + p.background_mean)
+    r = _emitter_scale(p, seen, cam[:, :, 1], png, E)
+    assert len(r) == 400
+    assert abs(numpy.median(r) - 1.0) < 0.005, numpy.median(r)
+    planted = p.QE * (1.01 * seen['camera'][:, :, 0] + p.background_mean)
+    rp = _emitter_scale(p, seen, cam[:, :, 1], png, planted)
+    assert abs(numpy.median(rp) - 1.0) > 0.005, numpy.median(rp)
