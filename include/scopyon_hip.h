@@ -166,6 +166,24 @@ int smx_splat_accumulate_ts(int n, const double* u, const double* v, const doubl
                             int m, double* out, void* workspace, size_t workspace_bytes,
                             int* status_dev, hipStream_t stream, int tile_side);
 
+/* Allocation-free form of smx_splat_expected_ts, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments without the workspace: the binning workspace lives in a fixed
+ * device array of smx_splat_static_scratch_bytes() (64 MiB) that each GPU the
+ * library runs on holds until the process ends; a call whose workspace
+ * (smx_splat_workspace_bytes(n, n_tiles, list_capacity)) exceeds it returns
+ * SMX_EVALUE.  Calls that overlap on one GPU share that array and race:
+ * order them on one stream.  Nothing here calls hipMalloc or hipFree. */
+size_t smx_splat_static_scratch_bytes(void);
+int smx_splat_expected_static(int n, const double* u, const double* v, const double* w,
+                              const int* key, const int* fov_id, const smx_fov_t* fovs_dev,
+                              int nfov, int n_tiles, const double* sat, size_t sat_stride,
+                              int n_slots, int ntab, int sat_period, double res, double unit_area,
+                              long long list_capacity, int accumulate, const void* K, int k_format,
+                              int m, double* out, int* status_dev, hipStream_t stream,
+                              int tile_side);
+
 /* ---- detector (_epifm.py:355-416, 1430-1484) ---------------------------- */
 /* E = QE * (photons + background) (_epifm.py:1436-1441); flag_out (may be
  * NULL) = E > 0, the pixels that consume a uniform in the EMCCD draw. */

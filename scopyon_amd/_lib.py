@@ -117,6 +117,11 @@ SIGNATURES['smx_splat_bin'] = SIGNATURES['smx_splat_expected']
 SIGNATURES['smx_splat_accumulate'] = SIGNATURES['smx_splat_expected']
 for _name in ('smx_splat_expected', 'smx_splat_bin', 'smx_splat_accumulate'):
     SIGNATURES[_name + '_ts'] = (c_int, SIGNATURES[_name][1] + [c_int])
+# smx_splat_expected_ts without (workspace, workspace_bytes)
+SIGNATURES['smx_splat_static_scratch_bytes'] = (c_size_t, [])
+SIGNATURES['smx_splat_expected_static'] = (
+    c_int, SIGNATURES['smx_splat_expected'][1][:22] + SIGNATURES['smx_splat_expected'][1][24:] +
+    [c_int])
 
 
 def lib():

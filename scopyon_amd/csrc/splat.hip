@@ -17,6 +17,7 @@
 // run-to-run deterministic), then one 256-thread workgroup per tile
 // accumulates its 1024 pixels in registers.  No float atomics.
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 #include "common.hpp"
 #include "scopyon_hip_types.h"
@@ -1218,6 +1219,88 @@ int smx_splat_bin_ts(SMX_SPLAT_PARAMS, int tile_side) {
 }
 int smx_splat_accumulate_ts(SMX_SPLAT_PARAMS, int tile_side) {
     return splat_run(SMX_SPLAT_ARGS, 2, tile_side);
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// Allocation-free splat: the binning workspace of smx_splat_expected_ts lives
+// in `scratch`, a fixed device array of SMX_SPLAT_SCRATCH_BYTES that every
+// GPU the library runs on holds until the process ends.  In host code the
+// array's name is only a stand-in: its device address on the current GPU
+// comes from hipGetSymbolAddress, once per device, and only that address is
+// handed to kernels and to the scan.  Every call on one GPU shares that GPU's
+// array, so calls that overlap on one GPU (two streams, two threads) race.
+// Nothing in the array must start at zero: each call re-initialises the tile
+// counters it reads.
+// --------------------------------------------------------------------------
+#define SMX_SPLAT_SCRATCH_BYTES (64ull << 20)
+#define SMX_SCRATCH_MAX_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[SMX_SPLAT_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SMX_SCRATCH_MAX_DEVICES];
+
+// Device address of `scratch` on the current GPU (looked up on the first call
+// for that GPU).
+int scratch_address(void** out) {
+    int dev = -1;
+    if (smx::check_hip(hipGetDevice(&dev), "smx_splat_expected_static: hipGetDevice"))
+        return SMX_ERUNTIME;
+    if (dev < 0 || dev >= SMX_SCRATCH_MAX_DEVICES) {
+        smx::set_error("smx_splat_expected_static: device %d out of range", dev);
+        return SMX_ERUNTIME;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            smx::set_error("smx_splat_expected_static: hipGetSymbolAddress(scratch) on device "
+                           "%d: %s", dev, hipGetErrorString(e));
+            return SMX_ERUNTIME;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t smx_splat_static_scratch_bytes(void) { return SMX_SPLAT_SCRATCH_BYTES; }
+
+int smx_splat_expected_static(int n, const double* u, const double* v, const double* w,
+                              const int* key, const int* fov_id, const smx_fov_t* fovs_dev,
+                              int nfov, int n_tiles, const double* sat, size_t sat_stride,
+                              int n_slots, int ntab, int sat_period, double res, double unit_area,
+                              long long list_capacity, int accumulate, const void* K, int k_format,
+                              int m, double* out, int* status_dev, hipStream_t stream,
+                              int tile_side) {
+    if (n < 0 || n_tiles <= 0 || list_capacity < 0) {
+        smx::set_error("smx_splat_expected_static: invalid arguments (n=%d, n_tiles=%d)", n,
+                       n_tiles);
+        return SMX_EVALUE;
+    }
+    const size_t need = layout_workspace(n, n_tiles, list_capacity, nullptr, nullptr);
+    if (need > SMX_SPLAT_SCRATCH_BYTES) {
+        smx::set_error("smx_splat_expected_static: %d emitters, %d tiles and %lld list entries "
+                       "need %zu bytes of scratch; the fixed array holds %llu (use "
+                       "smx_splat_expected_ts with a workspace)",
+                       n, n_tiles, list_capacity, need, SMX_SPLAT_SCRATCH_BYTES);
+        return SMX_EVALUE;
+    }
+    void* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != SMX_OK) return rc;
+    return splat_run(n, u, v, w, key, fov_id, fovs_dev, nfov, n_tiles, sat, sat_stride, n_slots,
+                     ntab, sat_period, res, unit_area, list_capacity, accumulate, K, k_format, m,
+                     out, ws, SMX_SPLAT_SCRATCH_BYTES, status_dev, stream, 3, tile_side);
 }
 
 __global__ void to_float_kernel(const double* __restrict__ a, size_t n, float* __restrict__ b) {
