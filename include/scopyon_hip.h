@@ -61,6 +61,21 @@ size_t smx_psf_sat_workspace_bytes(int nr, int nz);
 int smx_psf_sat_build(const double* radial, int nz, int nr, double res, int period,
                       double* workspace, double* sat, hipStream_t stream);
 
+/* Radial x depth lookup-table PSF (smx_lut_t, scopyon_hip_types.h): per
+ * depth slice z, the float32 pixel integrals K[qu][qv][du][dv]
+ * (smx_lut_table_elems floats) of the cartesian kernel built from the
+ * slice's float64 radial profile radial[z][k], k * r_max / (nb - 1), k < nb
+ * (smx_psf_radial_born_wolf with res = r_max / (nb - 1); the cartesian
+ * kernel of _epifm.py:98-126, the Born-Wolf profile of _epifm.py:180-213).
+ * Each entry integrates the kernel over the exact pixel square by the
+ * midpoint rule on cells of pixel_length / (q ns) (a float64 summed-area
+ * table per slice, in `workspace`, processed in as many slices at a time as
+ * it holds: smx_lut_build_workspace_bytes(lut, 1) per slice). */
+size_t smx_lut_table_elems(const smx_lut_t* lut /* host */);
+size_t smx_lut_build_workspace_bytes(const smx_lut_t* lut /* host */, int nz_batch);
+int smx_lut_build(const double* radial, int nz, int nb, const smx_lut_t* lut /* host */,
+                  double* workspace, size_t workspace_bytes, float* K, hipStream_t stream);
+
 /* ---- emitters and splat (_epifm.py:1267-1333, 215-282) ------------------ */
 /* Per-emitter weight w (the reference's `normalization`) and depth-table key
  * for one sub-step of length unit_time.  Replaces the scalar part of
@@ -165,6 +180,71 @@ int smx_splat_accumulate_ts(int n, const double* u, const double* v, const doubl
                             long long list_capacity, int accumulate, const void* K, int k_format,
                             int m, double* out, void* workspace, size_t workspace_bytes,
                             int* status_dev, hipStream_t stream, int tile_side);
+
+/* Splat and EMCCD detector in one pass over the tiles (phase tables K
+ * required, accumulate = 0): each tile's photons go from the tile pass's
+ * on-chip accumulators through the EMCCD register, readout noise and ADC
+ * (_epifm.py:1436-1469) of smx_emccd_philox -- the same Philox draws per
+ * pixel, so adc_out equals smx_splat_expected_ts followed by
+ * smx_emccd_philox(pixel count, out, gain_map, offset_map, params, seed,
+ * frame, pix_base, frame_dev, E_out, adc_out, adc_is_float32) bit for bit --
+ * without writing the photon image to memory and reading it back.  `out`
+ * (the float64 photons) may be NULL; when given it is written too.
+ * smx_splat_detect_accumulate_ts is the tile pass alone on an already binned
+ * workspace (the second half, like smx_splat_accumulate_ts).  No reference
+ * counterpart as one call (the reference forms the image, then runs the
+ * detector, _epifm.py:1199, 1220). */
+int smx_splat_detect_ts(int n, const double* u, const double* v, const double* w, const int* key,
+                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
+                        const double* sat, size_t sat_stride, int n_slots, int ntab,
+                        int sat_period, double res, double unit_area, long long list_capacity,
+                        int accumulate, const void* K, int k_format, int m, double* out,
+                        void* workspace, size_t workspace_bytes, int* status_dev,
+                        hipStream_t stream, const double* gain_map, const double* offset_map,
+                        const smx_detector_params_t* params /* host */, uint64_t seed,
+                        uint32_t frame, uint32_t pix_base, const uint32_t* frame_dev,
+                        double* E_out, void* adc_out, int adc_is_float32, int tile_side);
+int smx_splat_detect_accumulate_ts(int n, const double* u, const double* v, const double* w,
+                                   const int* key, const int* fov_id, const smx_fov_t* fovs_dev,
+                                   int nfov, int n_tiles, const double* sat, size_t sat_stride,
+                                   int n_slots, int ntab, int sat_period, double res,
+                                   double unit_area, long long list_capacity, int accumulate,
+                                   const void* K, int k_format, int m, double* out,
+                                   void* workspace, size_t workspace_bytes, int* status_dev,
+                                   hipStream_t stream, const double* gain_map,
+                                   const double* offset_map,
+                                   const smx_detector_params_t* params /* host */, uint64_t seed,
+                                   uint32_t frame, uint32_t pix_base, const uint32_t* frame_dev,
+                                   double* E_out, void* adc_out, int adc_is_float32,
+                                   int tile_side);
+
+/* Pixel integration in LUT mode: the overlay of _epifm.py:215-282 with the
+ * pixel integrals of lut_tables (smx_lut_build; key[i] = depth slice of
+ * emitter i) instead of 1 nm cell sums.  Emitter i at a = (W/2 + x) / pl
+ * pixels from the frame edge takes the bilinear combination of the four
+ * tabulated phases around a q.  Same tiles, binning, workspace
+ * (smx_splat_workspace_bytes), list order and status bits as
+ * smx_splat_expected_ts; stages: 1 bin, 2 tile pass on a binned workspace,
+ * 3 both.  smx_splat_lut_detect_ts fuses the EMCCD detector into the tile
+ * pass like smx_splat_detect_ts (accumulate = 0; out may be NULL).  No
+ * reference counterpart (a deliberate deviation: the reference keeps one
+ * 1 nm table per 1 nm depth key). */
+int smx_splat_lut_ts(int n, const double* u, const double* v, const double* w, const int* key,
+                     const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
+                     const float* lut_tables, const smx_lut_t* lut /* host */,
+                     long long list_capacity, int accumulate, double* out, void* workspace,
+                     size_t workspace_bytes, int* status_dev, hipStream_t stream, int stages,
+                     int tile_side);
+int smx_splat_lut_detect_ts(int n, const double* u, const double* v, const double* w,
+                            const int* key, const int* fov_id, const smx_fov_t* fovs_dev, int nfov,
+                            int n_tiles, const float* lut_tables, const smx_lut_t* lut /* host */,
+                            long long list_capacity, int accumulate, double* out, void* workspace,
+                            size_t workspace_bytes, int* status_dev, hipStream_t stream,
+                            const double* gain_map, const double* offset_map,
+                            const smx_detector_params_t* params /* host */, uint64_t seed,
+                            uint32_t frame, uint32_t pix_base, const uint32_t* frame_dev,
+                            double* E_out, void* adc_out, int adc_is_float32, int stages,
+                            int tile_side);
 
 /* Allocation-free form of smx_splat_expected_ts, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,

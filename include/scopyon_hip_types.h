@@ -72,6 +72,28 @@ typedef struct smx_gauss_params_t {
     int32_t radius;     /* window half-width R (1 <= R <= 7): 2R + 1 pixels per axis */
 } smx_gauss_params_t;
 
+/* Radial x depth lookup-table PSF ("LUT mode", a deliberate deviation from
+ * the reference's 1 nm depth keys and 1 nm cell sums, for 3D work at any
+ * pixel size).  Per depth slice, K[qu][qv][du][dv] (float32) holds the
+ * integral of the slice's PSF over footprint pixel (du, dv) when the
+ * emitter sits at sub-pixel phase (qu / q, qv / q) of its pixel: pixel du
+ * spans [du - c - qu / q, du - c + 1 - qu / q) pixels from the emitter.  The
+ * PSF is the reference's cartesian kernel: the radial profile P(min(r,
+ * r_max)) on the square |x|, |y| <= r_max, zero outside (_epifm.py:98-126),
+ * with P from an FP64 Born-Wolf table on nb radial bins (_epifm.py:180-213).
+ * Emitters take the bilinear combination of the four phases around their
+ * exact position. */
+typedef struct smx_lut_t {
+    int32_t q;             /* phase steps per pixel (the table holds q + 1 per axis) */
+    int32_t m;             /* footprint pixels per axis, 2 c + 1 */
+    int32_t c;             /* footprint pixel holding the emitter (floor(r_max / pl) + 1) */
+    int32_t nslices;       /* depth slices */
+    int32_t ns;            /* build: midpoint cells per phase step per axis */
+    int32_t reserved;
+    double pixel_length;   /* detector pixel / magnification [m] */
+    double r_max;          /* radial extent of the profile [m] */
+} smx_lut_t;
+
 #ifdef __cplusplus
 }
 #endif

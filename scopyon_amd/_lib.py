@@ -41,6 +41,13 @@ class GaussParamsT(ctypes.Structure):
                 ('background', ctypes.c_float), ('radius', ctypes.c_int32)]
 
 
+class LutT(ctypes.Structure):
+    """smx_lut_t."""
+    _fields_ = [('q', ctypes.c_int32), ('m', ctypes.c_int32), ('c', ctypes.c_int32),
+                ('nslices', ctypes.c_int32), ('ns', ctypes.c_int32), ('reserved', ctypes.c_int32),
+                ('pixel_length', c_double), ('r_max', c_double)]
+
+
 class DetectorParamsT(ctypes.Structure):
     """smx_detector_params_t."""
     _fields_ = [('qe', c_double), ('background', c_double), ('background_switch', ctypes.c_int32),
@@ -117,6 +124,22 @@ SIGNATURES['smx_splat_bin'] = SIGNATURES['smx_splat_expected']
 SIGNATURES['smx_splat_accumulate'] = SIGNATURES['smx_splat_expected']
 for _name in ('smx_splat_expected', 'smx_splat_bin', 'smx_splat_accumulate'):
     SIGNATURES[_name + '_ts'] = (c_int, SIGNATURES[_name][1] + [c_int])
+# smx_splat_expected + (gain_map, offset_map, params, seed, frame, pix_base,
+# frame_dev, E_out, adc_out, adc_is_float32, tile_side)
+for _name in ('smx_splat_detect_ts', 'smx_splat_detect_accumulate_ts'):
+    SIGNATURES[_name] = (c_int, SIGNATURES['smx_splat_expected'][1] + [
+        P, P, ctypes.POINTER(DetectorParamsT), c_uint64, c_uint32, c_uint32, P, P, P, c_int,
+        c_int])
+_LUT = ctypes.POINTER(LutT)
+SIGNATURES['smx_lut_table_elems'] = (c_size_t, [_LUT])
+SIGNATURES['smx_lut_build_workspace_bytes'] = (c_size_t, [_LUT, c_int])
+SIGNATURES['smx_lut_build'] = (c_int, [P, c_int, c_int, _LUT, P, c_size_t, P, P])
+_LUT_SPLAT = [c_int, P, P, P, P, P, P, c_int, c_int, P, _LUT, c_longlong, c_int, P, P, c_size_t,
+              P, P]
+SIGNATURES['smx_splat_lut_ts'] = (c_int, _LUT_SPLAT + [c_int, c_int])
+SIGNATURES['smx_splat_lut_detect_ts'] = (c_int, _LUT_SPLAT + [
+    P, P, ctypes.POINTER(DetectorParamsT), c_uint64, c_uint32, c_uint32, P, P, P, c_int, c_int,
+    c_int])
 # smx_splat_expected_ts without (workspace, workspace_bytes)
 SIGNATURES['smx_splat_static_scratch_bytes'] = (c_size_t, [])
 SIGNATURES['smx_splat_expected_static'] = (

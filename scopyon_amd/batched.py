@@ -27,7 +27,7 @@ import torch
 
 from . import _device, _lib
 from ._epifm import EPIFMConfigs, FrameGeometry, Splatter, tiles_per_emitter
-from ._psf import PSFTables
+from ._psf import LUTTables, PSFTables
 
 __all__ = ['BatchedSimulator']
 
@@ -37,7 +37,8 @@ class BatchedSimulator:
     def __init__(self, config, fov_shapes, n_emitters, *, D=0.1e-12, dt=None, seed=0,
                  boundary='reflect', box=None, depth_range=None, method='default',
                  adc_dtype=torch.float32, keep_expectation=False, table_precision='folded',
-                 tile_side='auto'):
+                 tile_side='auto', fuse_detector=True, keep_photons=False, psf='table',
+                 lut_slices=256, lut_bins=4096, lut_phases=32):
         self.device = _device.require_gpu()
         conf = config[method] if hasattr(config, '__getitem__') else config
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
@@ -60,22 +61,31 @@ class BatchedSimulator:
         self.tile_side = 32
         self._set_geometry(32)
         geoms = self.geoms
-        # PSF tables: depth 0 only for 2D batches, every key for 3D batches
-        self.tables = PSFTables(c.fluorophore_type, c.psf_wavelength, c.radial_cutoff,
-                                c.depth_cutoff, c.psf_radial_width, pl, self.device)
         if boundary not in ('none', 'periodic', 'reflect'):
             raise ValueError("boundary must be 'none', 'periodic' or 'reflect'")
-        if depth_range is None:
-            self.tables.ensure([0])
-        elif boundary == 'none':
-            # nothing keeps an emitter inside the depth range: every key up to
-            # the cutoff, so no emitter can meet an unbuilt table
-            self.tables.prebuild_all()
+        if psf not in ('table', 'lut'):
+            raise ValueError("psf must be 'table' (1 nm depth keys) or 'lut'")
+        self.lut = psf == 'lut'
+        if self.lut:
+            # radial x depth lookup table: every depth at once, any pixel size
+            self.tables = LUTTables(c.fluorophore_type, c.psf_wavelength, c.radial_cutoff,
+                                    c.depth_cutoff, c.psf_radial_width, pl, self.device,
+                                    nz=lut_slices, nb=lut_bins, q=lut_phases)
         else:
-            # every 1 nm key an emitter inside the depth range (walls or the
-            # periodic wrap keep it there) can reach, plus the cutoff key
-            dmax = max(abs(depth_range[0]), abs(depth_range[1]))
-            self.tables.prebuild_range(dmax)
+            # PSF tables: depth 0 only for 2D batches, every key for 3D batches
+            self.tables = PSFTables(c.fluorophore_type, c.psf_wavelength, c.radial_cutoff,
+                                    c.depth_cutoff, c.psf_radial_width, pl, self.device)
+            if depth_range is None:
+                self.tables.ensure([0])
+            elif boundary == 'none':
+                # nothing keeps an emitter inside the depth range: every key up
+                # to the cutoff, so no emitter can meet an unbuilt table
+                self.tables.prebuild_all()
+            else:
+                # every 1 nm key an emitter inside the depth range (walls or
+                # the periodic wrap keep it there) can reach, plus the cutoff key
+                dmax = max(abs(depth_range[0]), abs(depth_range[1]))
+                self.tables.prebuild_range(dmax)
         self.slot_map = self.tables.slot_map()
         self.splatter = Splatter(c, self.tables, self.device)
         self.det = c.detector_params()
@@ -140,9 +150,11 @@ class BatchedSimulator:
         self.key = torch.empty(self.n, dtype=torch.int32, device=self.device)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         # tiles of side >= 16: at most tiles_per_emitter(..., 16) per emitter
-        self.capacity = max(1, self.n * tiles_per_emitter(pl, self.tables.ntab))
+        per = ((self.tables.m // 16 + 2) ** 2 if self.lut
+               else tiles_per_emitter(pl, self.tables.ntab))
+        self.capacity = max(1, self.n * per)
         self._set_workspace()
-        self.use_phase = self.tables.use_phase
+        self.use_phase = (not self.lut) and self.tables.use_phase
         # 'float64': the phase tables themselves (same values as the SAT path);
         # 'float32': their float32 copy (half the table traffic); 'folded': the
         # symmetric float32 fold (1/8 of the table, L2-resident).  With float32
@@ -151,6 +163,8 @@ class BatchedSimulator:
             raise ValueError('table_precision must be float64, float32 or folded')
         self.k_format = 0
         self.Ktab = None
+        if self.lut:
+            self.Ktab = self.tables.K
         if self.use_phase:
             self.k_format = {'float64': 0, 'float32': 1, 'folded': 2}[table_precision]
             if self.k_format == 0:
@@ -159,13 +173,19 @@ class BatchedSimulator:
                 self.Ktab = self.tables.k32()
             else:
                 self.Ktab = self.tables.kfold()
+        # EMCCD frames on phase tables: the detector runs inside the tile pass
+        # (smx_splat_detect_ts), the photon image stays on chip unless
+        # keep_photons asks for it
+        self.fused = (bool(fuse_detector) and (self.use_phase or self.lut) and
+                      c.detector_type == 'EMCCD')
+        self.keep_photons = bool(keep_photons) or not self.fused
         self.graph = None
         self.tile_timings = {}
         if tile_side == 'auto':
-            if self.use_phase:
+            if self.use_phase or self.lut:
                 self.autotune_tiles()
         elif int(tile_side) != 32:
-            if int(tile_side) not in self.TILE_SIDES or not self.use_phase:
+            if int(tile_side) not in self.TILE_SIDES or not (self.use_phase or self.lut):
                 raise ValueError('tile_side must be 16, 32 or 64 (32 without phase tables)')
             self._set_geometry(int(tile_side))
             self._set_workspace()
@@ -252,6 +272,15 @@ class BatchedSimulator:
         halves smx_splat_bin / smx_splat_accumulate), in tiles of side
         `tile_side`."""
         t = self.tables
+        if self.lut:
+            stages = {'smx_splat_expected': 3, 'smx_splat_bin': 1, 'smx_splat_accumulate': 2}
+            _lib.call('smx_splat_lut_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
+                      self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
+                      self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.K.data_ptr(), t.lut,
+                      self.capacity, 0, self.photons.data_ptr(), self.ws.data_ptr(),
+                      self.ws.numel(), self.splatter.status.data_ptr(), _device.stream(),
+                      stages[entry], self.tile_side)
+            return
         _lib.call(entry + '_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
                   self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.sat.data_ptr(),
@@ -263,7 +292,51 @@ class BatchedSimulator:
     def detect(self):
         """EMCCD + ADC of every frame, then advance the frame counter."""
         self.emccd()
+        self.advance()
+
+    def advance(self):
+        """Advance the device frame counter (the Philox frame of the next step)."""
         _lib.call('smx_counter_add', self.counter.data_ptr(), 1, _device.stream())
+
+    def splat_detect(self, entry='smx_splat_detect_ts'):
+        """Splat and EMCCD + ADC of every frame in one tile pass (the emitter
+        weights must be current): same ADC as splat() then emccd(), without
+        the photon image's round trip through memory (it is written only with
+        keep_photons).  entry: smx_splat_detect_ts, or
+        smx_splat_detect_accumulate_ts on an already binned workspace."""
+        t = self.tables
+        f32 = 1 if self.adc.dtype == torch.float32 else 0
+        out = self.photons.data_ptr() if self.keep_photons else None
+        if self.lut:
+            stages = {'smx_splat_detect_ts': 3, 'smx_splat_detect_accumulate_ts': 2}
+            _lib.call('smx_splat_lut_detect_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
+                      self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
+                      self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.K.data_ptr(), t.lut,
+                      self.capacity, 0, out, self.ws.data_ptr(), self.ws.numel(),
+                      self.splatter.status.data_ptr(), _device.stream(), None, None, self.det,
+                      self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
+                      self.adc.data_ptr(), f32, stages[entry], self.tile_side)
+            return
+        _lib.call(entry, self.n, self.u.data_ptr(), self.v.data_ptr(), self.w.data_ptr(),
+                  self.key.data_ptr(), self.fov_id.data_ptr(), self.fovs_dev.data_ptr(),
+                  self.nfov, self.n_tiles, t.sat.data_ptr(), t.sat_elems, len(t.slot_of), t.ntab,
+                  t.period, 1e-9, 1e-18, self.capacity, 0, _device.ptr(self.Ktab), self.k_format,
+                  t.m, out, self.ws.data_ptr(), self.ws.numel(),
+                  self.splatter.status.data_ptr(), _device.stream(), None, None, self.det,
+                  self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
+                  self.adc.data_ptr(), f32, self.tile_side)
+
+    def stages(self):
+        """The launches of one step, in order, as callables (for timing)."""
+        st = {'brownian_philox': self.move, 'emitter_weights': self.weights,
+              'splat_bin': lambda: self.splat('smx_splat_bin')}
+        if self.fused:
+            st['splat_detect'] = lambda: self.splat_detect('smx_splat_detect_accumulate_ts')
+        else:
+            st['splat_accumulate'] = lambda: self.splat('smx_splat_accumulate')
+            st['emccd_philox'] = self.emccd
+        st['counter_add'] = self.advance
+        return st
 
     def emccd(self):
         """Detector + ADC of every pixel (EMCCD, or CCD / CMOS by configuration)."""
@@ -287,8 +360,13 @@ class BatchedSimulator:
     def step(self):
         """Advance every FOV by one time step and form + detect its frame."""
         self.move()
-        self.form()
-        self.detect()
+        if self.fused:
+            self.weights()
+            self.splat_detect()
+            self.advance()
+        else:
+            self.form()
+            self.detect()
 
     # ---- HIP graph ----
     def capture(self):

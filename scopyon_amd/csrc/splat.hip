@@ -20,6 +20,7 @@
 #include <mutex>
 #include <type_traits>
 #include "common.hpp"
+#include "emccd_block.hpp"
 #include "scopyon_hip_types.h"
 #include "tiles.hpp"
 
@@ -382,6 +383,7 @@ struct EmitterRec {
     short n_u, n_v;          // footprint pixels per axis (hi - lo)
     short su, sv;            // table element strides per footprint pixel along u, v
     int flags;               // 1: irregular boundaries -> SAT path; 2: no table slot
+    unsigned frac;           // LUT mode: sub-phase fractions along u (low 16 bits) and v
     double w;
     long long table_off;     // element of pixel (du, dv) = table_off + du * su + dv * sv
 };
@@ -440,6 +442,7 @@ __device__ inline EmitterRec make_rec(int i, int f, const double* __restrict__ u
     r.su = (short)M;
     r.sv = 1;
     r.flags = 0;
+    r.frac = 0;
     r.w = w[i];
     r.table_off = 0;
     int slot = key[i];
@@ -513,13 +516,69 @@ __device__ inline int window_base(const int* __restrict__ fov_id, const smx_fov_
     return fov_ok(f0, nfov) ? fovs[f0].tile_off : 0;
 }
 
+// LUT-mode geometry of the records (q = 0: phase tables of the 1 nm path).
+struct LutRec {
+    int q, m, c;
+    size_t slice_elems;
+};
+
+// LUT-mode record of emitter i in FOV f: position a = (W/2 + x) / pl in
+// pixels from the frame edge (pixel k spans [k, k + 1), _epifm.py:230-231),
+// a q = q_i + frac: phase q_i mod q of pixel floor(q_i / q), the footprint
+// starts c pixels before it; the table block of phases (q_u, q_v) and the
+// 16-bit fractions for the bilinear taps towards (q_u + 1, q_v + 1).
+__device__ inline EmitterRec make_rec_lut(int i, int f, const double* __restrict__ u,
+                                          const double* __restrict__ v,
+                                          const double* __restrict__ w, const int* __restrict__ key,
+                                          const smx_fov_t* __restrict__ fovs, int nfov, int n_slots,
+                                          const LutRec& LR, int* __restrict__ status) {
+    EmitterRec r;
+    r.lo_u = r.lo_v = 0;
+    r.n_u = r.n_v = 0;
+    r.su = (short)LR.m;
+    r.sv = 1;
+    r.flags = 0;
+    r.frac = 0;
+    r.w = w[i];
+    r.table_off = 0;
+    const int slot = key[i];
+    if (!fov_ok(f, nfov) || slot < 0 || slot >= n_slots) {
+        r.flags = 2;
+        if (fov_ok(f, nfov) && r.w > 0.0) atomicOr(status, 8);
+        return r;
+    }
+    const smx_fov_t& F = fovs[f];
+    int ph[2];
+    unsigned fr[2];
+    for (int ax = 0; ax < 2; ++ax) {
+        const double half = ax == 0 ? F.half_w : F.half_h;
+        const double x = ax == 0 ? u[i] - F.p0u : v[i] - F.p0v;
+        const double t = (half + x) / F.pixel_length * (double)LR.q;
+        ph[ax] = 0;
+        fr[ax] = 0;
+        if (!(fabs(t) < 1.0e9)) continue;        // far off the frame (or NaN): nothing binned
+        const double qi = floor(t);
+        const long long qq = (long long)qi;
+        long long pix = qq / LR.q;
+        if (qq - pix * LR.q < 0) --pix;          // floor division
+        ph[ax] = (int)(qq - pix * LR.q);
+        fr[ax] = (unsigned)((t - qi) * 65535.0 + 0.5);
+        const int lo = (int)(pix - LR.c);
+        if (ax == 0) { r.lo_u = lo; r.n_u = (short)LR.m; } else { r.lo_v = lo; r.n_v = (short)LR.m; }
+    }
+    r.frac = fr[0] | (fr[1] << 16);
+    r.table_off = (long long)slot * (long long)LR.slice_elems +
+                  ((long long)ph[0] * (LR.q + 1) + ph[1]) * (long long)(LR.m * LR.m);
+    return r;
+}
+
 __global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
         int n, const double* __restrict__ u, const double* __restrict__ v,
         const double* __restrict__ w, const int* __restrict__ key, const int* __restrict__ fov_id,
         const smx_fov_t* __restrict__ fovs, int nfov, double ext_half, double res, int ntab, int P,
         int M, int n_slots, size_t k_stride, int folded, smx::PhaseFold PF,
         EmitterRec* __restrict__ rec, int n_tiles, int* __restrict__ tile_count, int cs,
-        int* __restrict__ status, int lts) {
+        int* __restrict__ status, int lts, LutRec LR) {
     __shared__ int s_hist[REC_WIN];
     smx::bin_hist_clear(s_hist);
     const int wbase = window_base(fov_id, fovs, nfov, n);
@@ -527,8 +586,10 @@ __global__ void __launch_bounds__(REC_THREADS) emitter_rec_kernel(
     const int i = blockIdx.x * REC_THREADS + threadIdx.x;
     if (i < n) {
         const int f = fov_of(fov_id, i);
-        const EmitterRec r = make_rec(i, f, u, v, w, key, fovs, nfov, ext_half, res, ntab, P, M,
-                                      n_slots, k_stride, folded, PF, status);
+        const EmitterRec r =
+                LR.q > 0 ? make_rec_lut(i, f, u, v, w, key, fovs, nfov, n_slots, LR, status)
+                         : make_rec(i, f, u, v, w, key, fovs, nfov, ext_half, res, ntab, P, M,
+                                    n_slots, k_stride, folded, PF, status);
         rec[i] = r;
         TileRange t;
         if (tile_count && r.flags != 2 && r.w > 0.0 && rec_tiles(fovs[f], r, t, lts)) {
@@ -671,6 +732,21 @@ struct FastArgs {
     int accumulate;
     double* out;
     int* status;
+    int tap_u, tap_v;        // LUT mode: table bytes to the next phase along u, v
+};
+
+// Detector epilogue of the fused tile pass (EMCCD + readout noise + ADC of
+// the tile's pixels straight from the LDS accumulators).
+struct DetectArgs {
+    smx_detector_params_t P;
+    const double* gain_map;      // NULL: P.gain
+    const double* offset_map;    // NULL: P.offset
+    uint64_t seed;
+    uint32_t frame, pix_base;
+    const uint32_t* frame_dev;   // NULL, or added to frame
+    double* E_out;               // NULL, or QE * (photons + background)
+    void* adc_out;
+    int adc_f32;                 // 1: float ADC, 0: double
 };
 
 
@@ -745,24 +821,63 @@ constexpr int PMAX = 64;          // parts per wave per round (one lane writes e
 #define SMX_SPLAT_WPE 6
 #endif
 
-template <typename KT, int QPW, bool O32>
-__global__ void __launch_bounds__(64 * QPW * QPW) __attribute__((amdgpu_waves_per_eu(SMX_SPLAT_WPE, 8)))
-splat_phase_kernel(FastArgs A) {
-    constexpr int WPT = QPW * QPW, TSZ = QS * QPW, PSORT_CAP = psort_cap<QPW>();
-    constexpr int ACC = QS * AS;
+// LDS plan of the tile pass (byte offsets into one buffer).  With the
+// detector epilogue (DET) the EMCCD work queues of emccd_block take the same
+// buffer once the accumulators are read: from offset 0 for tiles of up to
+// 32 x 32 pixels (one round), after the accumulators for 64 x 64 tiles
+// (four rounds of 1024 pixels that read the accumulators in turn).
+template <int QPW, bool DET, int TAPS>
+struct SplatLds {
+    static constexpr int WPT = QPW * QPW, TSZ = QS * QPW, ACC = QS * AS;
     // accumulators, then one spill slot per lane shared by the waves (lanes
     // outside a part add there: no branch per part, the slots are never read);
     // before the accumulators are cleared the same space sorts the tile list
-    constexpr int ACCN = WPT * ACC + 64 > PSORT_CAP / 2 ? WPT * ACC + 64 : PSORT_CAP / 2;
-    __shared__ double s_acc[ACCN];
+    static constexpr int ACCN = WPT * ACC + 64 > psort_cap<QPW>() / 2 ? WPT * ACC + 64
+                                                                       : psort_cap<QPW>() / 2;
+    static constexpr int GEO = (ACCN * 8 + 15) & ~15;   // int4 lo_u, lo_v (frame pixels), n_u, n_v
+    static constexpr int STR = GEO + FCHUNK * 16;       // int2 table strides per pixel along u, v
+    static constexpr int FLAGS = STR + FCHUNK * 8;
+    static constexpr int EID = FLAGS + FCHUNK * 4;
+    static constexpr int W = EID + FCHUNK * 4;          // double emitter weights
+    static constexpr int KO = W + FCHUNK * 8;           // table element of footprint pixel (0, 0)
+    static constexpr int PR = KO + FCHUNK * 8;          // PartRec [WPT][PMAX]: parts of one round
+    static constexpr int PW = PR + WPT * PMAX * 32;     // [WPT][PMAX] weights: double, or
+                                                        // float4 tap coefficients (TAPS 4)
+    static constexpr int FRAC = PW + WPT * PMAX * (TAPS == 4 ? 16 : 8);   // LUT fractions
+    static constexpr int SPLAT_END = FRAC + (TAPS == 4 ? FCHUNK * 4 : 0);
+    // detector epilogue: rounds of RP pixels, PPT per thread
+    static constexpr int NT = 64 * WPT;
+    static constexpr int RP = TSZ * TSZ < 1024 ? TSZ * TSZ : 1024;
+    static constexpr int PPT = RP / NT;
+    static constexpr int ROUNDS = TSZ * TSZ / RP;
+    static constexpr int QOFF = ROUNDS == 1 ? 0 : GEO;
+    static constexpr int Q_END = QOFF + smx::emccd::lds_bytes(RP);
+    static constexpr int BYTES = DET && Q_END > SPLAT_END ? Q_END : SPLAT_END;
+};
+
+template <typename KT, int QPW, bool O32, bool DET, int TAPS>
+__global__ void __launch_bounds__(64 * QPW * QPW) __attribute__((amdgpu_waves_per_eu(SMX_SPLAT_WPE, 8)))
+splat_phase_kernel(FastArgs A, DetectArgs D) {
+    using LP = SplatLds<QPW, DET, TAPS>;
+    // 64-pixel parts in flight per group: NSLOT table loads, or 4 parts of 4
+    // taps each in LUT mode
+    constexpr int NS = TAPS == 4 ? 4 : NSLOT;
+    constexpr int WPT = LP::WPT, TSZ = LP::TSZ, PSORT_CAP = psort_cap<QPW>();
+    constexpr int ACC = LP::ACC, ACCN = LP::ACCN;
+    __shared__ __align__(16) char s_lds[LP::BYTES];
+    double* const s_acc = (double*)s_lds;
     int* const s_sort = (int*)s_acc;
-    __shared__ int4 s_geo[FCHUNK];            // lo_u, lo_v (frame pixels), n_u, n_v
-    __shared__ int2 s_str[FCHUNK];            // table strides per pixel along u, v
-    __shared__ int s_flags[FCHUNK], s_e[FCHUNK];
-    __shared__ double s_w[FCHUNK];
-    __shared__ long long s_ko[FCHUNK];        // table element of footprint pixel (0, 0)
-    __shared__ PartRec s_pr[WPT][PMAX];       // per wave: the parts of one round
-    __shared__ double s_pw[WPT][PMAX];        // and their emitter weights
+    int4* const s_geo = (int4*)(s_lds + LP::GEO);
+    int2* const s_str = (int2*)(s_lds + LP::STR);
+    int* const s_flags = (int*)(s_lds + LP::FLAGS);
+    int* const s_e = (int*)(s_lds + LP::EID);
+    double* const s_w = (double*)(s_lds + LP::W);
+    long long* const s_ko = (long long*)(s_lds + LP::KO);
+    PartRec (*const s_pr)[PMAX] = (PartRec (*)[PMAX])(s_lds + LP::PR);
+    double (*const s_pw)[PMAX] = (double (*)[PMAX])(s_lds + LP::PW);
+    float4 (*const s_pc)[PMAX] = (float4 (*)[PMAX])(s_lds + LP::PW);
+    unsigned* const s_frac = (unsigned*)(s_lds + LP::FRAC);
+    (void)ACCN;
 
     const int tile = blockIdx.x;
     const int fov = A.tile_fov[tile];
@@ -816,6 +931,7 @@ splat_phase_kernel(FastArgs A) {
             s_str[threadIdx.x] = make_int2(r.su, r.sv);
             s_flags[threadIdx.x] = r.flags;
             s_w[threadIdx.x] = r.w;
+            if (TAPS == 4) s_frac[threadIdx.x] = r.frac;
             s_ko[threadIdx.x] = r.flags == 0 ? r.table_off : 0;
             if (r.flags & 2) atomicOr(A.status, 8);
         }
@@ -880,8 +996,19 @@ splat_phase_kernel(FastArgs A) {
             } else {
                 PartRec* pr = s_pr[wave];
                 double* pw = s_pw[wave];
+                float4* pc = s_pc[wave];
                 const int magic = (65536 + W - 1) / W;
                 const double w = lane < nc ? s_w[lane] : 0.0;
+                // LUT mode: bilinear weights of the four phase taps, times w
+                float4 cw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (TAPS == 4 && lane < nc) {
+                    const unsigned fr = s_frac[lane];
+                    const float fu = (float)(fr & 0xffffu) * (1.0f / 65535.0f);
+                    const float fv = (float)(fr >> 16) * (1.0f / 65535.0f);
+                    const float wf = (float)w;
+                    cw = make_float4(((1.0f - fu) * (1.0f - fv)) * wf, ((1.0f - fu) * fv) * wf,
+                                     (fu * (1.0f - fv)) * wf, (fu * fv) * wf);
+                }
                 const long long kb = kq * (long long)sizeof(KT);
                 const unsigned long long ka = (unsigned long long)(kbase + kb);
                 const int pix = acc_off + (rr0 * AS + cc0) * 8;
@@ -901,10 +1028,11 @@ splat_phase_kernel(FastArgs A) {
                         R.sv = sv * (int)sizeof(KT);
                         R.pix = pix;
                         R.kq_hi = O32 ? 0 : (int)(unsigned)(ka >> 32);
-                        pw[pos] = w;
+                        if (TAPS == 4) pc[pos] = cw;
+                        else pw[pos] = w;
                     }
                     const int cnt = min(PMAX, total - rb);
-                    const int cntp = (cnt + NSLOT - 1) & ~(NSLOT - 1);
+                    const int cntp = (cnt + NS - 1) & ~(NS - 1);
                     if (lane >= cnt && lane < cntp) {
                         PartRec& R = pr[lane];
                         R.kq = O32 ? 0 : (int)(unsigned)(unsigned long long)kbase;
@@ -915,42 +1043,60 @@ splat_phase_kernel(FastArgs A) {
                         R.sv = 0;
                         R.pix = 0;
                         R.kq_hi = O32 ? 0 : (int)(unsigned)((unsigned long long)kbase >> 32);
-                        pw[lane] = 0.0;
+                        if (TAPS == 4) pc[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        else pw[lane] = 0.0;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    for (int b = 0; b < cntp; b += NSLOT) {
-                        KT kv[NSLOT];
-                        double wv[NSLOT];
-                        unsigned la[NSLOT];
+                    for (int b = 0; b < cntp; b += NS) {
+                        KT kv[NS][TAPS];
+                        double wv[NS];
+                        unsigned la[NS];
 #pragma unroll
-                        for (int s = 0; s < NSLOT; ++s) {
+                        for (int s = 0; s < NS; ++s) {
                             const PartRec& R = pr[b + s];
                             const int4 h = *(const int4*)&R.kq;
                             const int4 g = *(const int4*)&R.su;
-                            wv[s] = pw[b + s];
+                            if (TAPS == 1) wv[s] = pw[b + s];
                             const unsigned idx = (unsigned)lane + (unsigned)h.w;
                             const bool ok = lane24 < (unsigned)h.y;
                             const int row = (int)(__umul24(idx, (unsigned)h.y) >> 16);
                             const int col = __mul24(row, h.z) + (int)idx;
+                            const char* kp;
                             if (O32) {
                                 int off = __mul24(col, g.y) + (__mul24(row, g.x) + h.x);
                                 off = ok ? off : 0;
-                                kv[s] = smx::gload((const KT*)(kbase + (unsigned)off));
+                                kp = kbase + (unsigned)off;
                             } else {
                                 int off = __mul24(col, g.y) + __mul24(row, g.x);
                                 off = ok ? off : 0;
-                                const char* kp = (const char*)(((unsigned long long)(unsigned)g.w << 32) |
-                                                               (unsigned)h.x);
-                                kv[s] = smx::gload((const KT*)(kp + off));
+                                kp = (const char*)(((unsigned long long)(unsigned)g.w << 32) |
+                                                   (unsigned)h.x) + off;
+                            }
+                            kv[s][0] = smx::gload((const KT*)kp);
+                            if (TAPS == 4) {
+                                kv[s][TAPS > 1 ? 1 : 0] = smx::gload((const KT*)(kp + A.tap_v));
+                                kv[s][TAPS > 2 ? 2 : 0] = smx::gload((const KT*)(kp + A.tap_u));
+                                kv[s][TAPS > 3 ? 3 : 0] =
+                                        smx::gload((const KT*)(kp + (A.tap_u + A.tap_v)));
                             }
                             const unsigned l = (unsigned)__mul24(row, AS * 8) + (unsigned)g.z + ((unsigned)col << 3);
                             la[s] = ok ? l : spill;
                         }
 #pragma unroll
-                        for (int s = 0; s < NSLOT; ++s)
-                            lds_add((double*)((char*)s_acc + la[s]), (double)kv[s] * wv[s]);
+                        for (int s = 0; s < NS; ++s) {
+                            if (TAPS == 4) {
+                                const float4 c = pc[b + s];
+                                const float val = ((c.x * (float)kv[s][0] +
+                                                    c.y * (float)kv[s][TAPS > 1 ? 1 : 0]) +
+                                                   c.z * (float)kv[s][TAPS > 2 ? 2 : 0]) +
+                                                  c.w * (float)kv[s][TAPS > 3 ? 3 : 0];
+                                lds_add((double*)((char*)s_acc + la[s]), (double)val);
+                            } else {
+                                lds_add((double*)((char*)s_acc + la[s]), (double)kv[s][0] * wv[s]);
+                            }
+                        }
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -960,18 +1106,70 @@ splat_phase_kernel(FastArgs A) {
         }
         __syncthreads();
     }
-    if (active) {
-        for (int k = lane; k < QS * QS; k += 64) {
-            const int r = k / QS, c = k % QS;
-            const double a = acc[r * AS + c];
-            if (r >= qh || c >= qw) continue;
-            const size_t o = (size_t)F.pix_off + (size_t)(qr0 + r) * F.nh + (qc0 + c);
-            if (A.accumulate) {
-                if (count > 0) A.out[o] = A.out[o] + a;
-            } else {
-                // streaming store: keep the phase table, not the frame, in L2
-                __builtin_nontemporal_store(a, A.out + o);
+    if (!DET) {
+        if (active) {
+            for (int k = lane; k < QS * QS; k += 64) {
+                const int r = k / QS, c = k % QS;
+                const double a = acc[r * AS + c];
+                if (r >= qh || c >= qw) continue;
+                const size_t o = (size_t)F.pix_off + (size_t)(qr0 + r) * F.nh + (qc0 + c);
+                if (A.accumulate) {
+                    if (count > 0) A.out[o] = A.out[o] + a;
+                } else {
+                    // streaming store: keep the phase table, not the frame, in L2
+                    __builtin_nontemporal_store(a, A.out + o);
+                }
             }
+        }
+        return;
+    }
+    // Detector epilogue: the tile's photons go from the accumulators through
+    // the EMCCD register, readout noise and ADC (_epifm.py:1436-1469) without
+    // a round trip through HBM; the draws are the standalone detector's
+    // (emccd_block, same Philox stream per pixel).
+    __syncthreads();   // every quadrant's accumulators complete (also when count == 0)
+    constexpr int NT = LP::NT, RP = LP::RP, PPT = LP::PPT;
+    const smx::emccd::BlockLds L = smx::emccd::carve(s_lds + LP::QOFF, RP);
+    const uint32_t frame = D.frame + (D.frame_dev ? *D.frame_dev : 0u);
+    const uint32_t k0 = (uint32_t)D.seed, k1 = (uint32_t)(D.seed >> 32);
+    const int t = threadIdx.x;
+    const long long row0 = (long long)tr * TSZ, col0 = (long long)tc * TSZ;
+    for (int rd = 0; rd < LP::ROUNDS; ++rd) {
+        float E[PPT];
+        bool in[PPT];
+        long long o[PPT];
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            const int p = rd * RP + t + j * NT;
+            const int r = p / TSZ, c = p % TSZ;
+            in[j] = row0 + r < F.nw && col0 + c < F.nh;
+            o[j] = F.pix_off + (row0 + r) * F.nh + (col0 + c);
+            E[j] = 0.0f;
+            const double a = s_acc[((r >> 4) * QPW + (c >> 4)) * ACC + (r & 15) * AS + (c & 15)];
+            if (in[j]) {
+                E[j] = smx::emccd::expectation_f32(a, D.P);
+                if (A.out) __builtin_nontemporal_store(a, A.out + o[j]);
+                if (D.E_out) D.E_out[o[j]] = (double)E[j];
+            }
+        }
+        if (LP::QOFF == 0) __syncthreads();   // accumulators read before the queues overlay them
+        float S[PPT], noise[PPT];
+        smx::emccd::emccd_block<NT, PPT>(
+                L, E, in,
+                [&](int loc) {
+                    const int p = rd * RP + loc;
+                    return D.pix_base + (uint32_t)(F.pix_off + (row0 + p / TSZ) * F.nh +
+                                                   (col0 + p % TSZ));
+                },
+                frame, k0, k1, D.P, S, noise);
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) {
+            if (!in[j]) continue;
+            const float gain = (float)(D.gain_map ? D.gain_map[o[j]] : D.P.gain);
+            const float offset = (float)(D.offset_map ? D.offset_map[o[j]] : D.P.offset);
+            const float x = smx::emccd::adc_f32(S[j], noise[j], D.P, gain, offset);
+            if (D.adc_f32) ((float*)D.adc_out)[o[j]] = x;
+            else ((double*)D.adc_out)[o[j]] = (double)x;
         }
     }
 }
@@ -1080,7 +1278,27 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                      double res, double unit_area, long long list_capacity, int accumulate,
                      const void* K, int k_format, int m, double* out, void* workspace,
                      size_t workspace_bytes, int* status_dev, hipStream_t stream, int stages,
-                     int tile_side) {
+                     int tile_side, const DetectArgs* det = nullptr,
+                     const smx_lut_t* lut = nullptr) {
+    LutRec LR{0, 0, 0, 0};
+    if (lut) {
+        // LUT mode: float32 pixel-integral tables per depth slice, four taps
+        if (!K || lut->q < 1 || lut->m != 2 * lut->c + 1 || lut->c < 1 || lut->nslices < 1) {
+            smx::set_error("smx_splat_lut: invalid table description");
+            return SMX_EVALUE;
+        }
+        LR.q = lut->q;
+        LR.m = lut->m;
+        LR.c = lut->c;
+        LR.slice_elems = (size_t)(lut->q + 1) * (lut->q + 1) * lut->m * lut->m;
+        sat = (const double*)K;    // not read: LUT records are never integrated through a SAT
+        sat_stride = 0;
+        n_slots = lut->nslices;
+        ntab = 1;
+        sat_period = 2;
+        m = lut->m;
+        k_format = 1;
+    }
     const smx::PhaseFold PF = smx::make_phase_fold(ntab, sat_period > 0 ? sat_period : 1);
     const int lts = tile_side == 16 ? 4 : tile_side == 32 ? 5 : tile_side == 64 ? 6 : -1;
     if (lts < 0 || (!K && lts != 5)) {
@@ -1088,7 +1306,8 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                        "32 without)", tile_side);
         return SMX_EVALUE;
     }
-    if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev || !out ||
+    if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev ||
+        (!out && !det) ||
         (K && (sat_period < 2 || m < 1 || k_format < 0 || k_format > 2)) ||
         (K && k_format == 2 && PF.mf > m) ||
         !workspace || !status_dev || (n > 0 && (!u || !v || !w || !key || !sat))) {
@@ -1123,7 +1342,7 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                                    sat_period, m, n_slots,
                                    folded ? PF.elems() : smx_phase_table_elems(sat_period, m),
                                    folded, PF, ws.rec, n_tiles, ws.tile_count, cs, status_dev,
-                                   lts);
+                                   lts, LR);
                 SMX_CHECK_LAUNCH("emitter_rec_kernel");
             } else {
                 hipLaunchKernelGGL(bin_count_kernel, dim3(nb), dim3(256), 0, stream, n, u, v, w,
@@ -1161,26 +1380,44 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         B.sat = sat; B.sat_stride = sat_stride; B.L = smx::make_sat_layout(ntab + 1, sat_period);
         B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
         B.accumulate = accumulate; B.out = out; B.status = status_dev;
+        B.tap_u = LR.q ? (LR.q + 1) * LR.m * LR.m * 4 : 0;
+        B.tap_v = LR.q ? LR.m * LR.m * 4 : 0;
         const dim3 grid((unsigned)n_tiles);
         // table byte offsets fit 32 bits: parts address it from its base
-        const size_t slot_elems = k_format == 2 ? PF.elems() : smx_phase_table_elems(sat_period, m);
+        const size_t slot_elems = LR.q ? LR.slice_elems
+                                : k_format == 2 ? PF.elems() : smx_phase_table_elems(sat_period, m);
         const double table_bytes = (double)n_slots * (double)slot_elems * (k_format != 0 ? 4.0 : 8.0);
         // SMX_SPLAT_ADDR64=1 takes the 64-bit form for any table (tests of that form)
         const char* a64 = getenv("SMX_SPLAT_ADDR64");
         const bool o32 = table_bytes < 4294967296.0 && !(a64 && a64[0] == '1');
-        auto launch = [&](auto kt, auto o) {
+        DetectArgs Dv{};
+        if (det) Dv = *det;
+        auto launch = [&](auto kt, auto o, auto dt, auto taps) {
             using KT = decltype(kt);
-            constexpr bool O = decltype(o)::value;
-            if (lts == 4) hipLaunchKernelGGL((splat_phase_kernel<KT, 1, O>), grid, dim3(64), 0, stream, B);
-            else if (lts == 5) hipLaunchKernelGGL((splat_phase_kernel<KT, 2, O>), grid, dim3(256), 0, stream, B);
-            else hipLaunchKernelGGL((splat_phase_kernel<KT, 4, O>), grid, dim3(1024), 0, stream, B);
+            constexpr bool O = decltype(o)::value, DT = decltype(dt)::value;
+            constexpr int TP = decltype(taps)::value;
+            if (lts == 4)
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 1, O, DT, TP>), grid, dim3(64), 0, stream, B, Dv);
+            else if (lts == 5)
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 2, O, DT, TP>), grid, dim3(256), 0, stream, B, Dv);
+            else
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 4, O, DT, TP>), grid, dim3(1024), 0, stream, B, Dv);
         };
-        if (k_format != 0) {
-            if (o32) launch(0.0f, std::true_type());
-            else launch(0.0f, std::false_type());
+        using One = std::integral_constant<int, 1>;
+        using Four = std::integral_constant<int, 4>;
+        auto launch_d = [&](auto kt, auto o, auto taps) {
+            if (det) launch(kt, o, std::true_type(), taps);
+            else launch(kt, o, std::false_type(), taps);
+        };
+        if (LR.q) {
+            if (o32) launch_d(0.0f, std::true_type(), Four());
+            else launch_d(0.0f, std::false_type(), Four());
+        } else if (k_format != 0) {
+            if (o32) launch_d(0.0f, std::true_type(), One());
+            else launch_d(0.0f, std::false_type(), One());
         } else {
-            if (o32) launch(0.0, std::true_type());
-            else launch(0.0, std::false_type());
+            if (o32) launch_d(0.0, std::true_type(), One());
+            else launch_d(0.0, std::false_type(), One());
         }
         SMX_CHECK_LAUNCH("splat_phase_kernel");
         return SMX_OK;
@@ -1219,6 +1456,90 @@ int smx_splat_bin_ts(SMX_SPLAT_PARAMS, int tile_side) {
 }
 int smx_splat_accumulate_ts(SMX_SPLAT_PARAMS, int tile_side) {
     return splat_run(SMX_SPLAT_ARGS, 2, tile_side);
+}
+
+static int splat_detect_run(SMX_SPLAT_PARAMS, const double* gain_map, const double* offset_map,
+                            const smx_detector_params_t* params, uint64_t seed, uint32_t frame,
+                            uint32_t pix_base, const uint32_t* frame_dev, double* E_out,
+                            void* adc_out, int adc_is_float32, int tile_side, int stages) {
+    if (!K || accumulate != 0 || !params || params->emgain <= 0 || !adc_out) {
+        smx::set_error("smx_splat_detect: needs phase tables (K), accumulate = 0, detector "
+                       "parameters with emgain > 0 and an ADC buffer");
+        return SMX_EVALUE;
+    }
+    DetectArgs d;
+    d.P = *params;
+    d.gain_map = gain_map;
+    d.offset_map = offset_map;
+    d.seed = seed;
+    d.frame = frame;
+    d.pix_base = pix_base;
+    d.frame_dev = frame_dev;
+    d.E_out = E_out;
+    d.adc_out = adc_out;
+    d.adc_f32 = adc_is_float32 ? 1 : 0;
+    return splat_run(SMX_SPLAT_ARGS, stages, tile_side, &d);
+}
+
+int smx_splat_detect_ts(SMX_SPLAT_PARAMS, const double* gain_map, const double* offset_map,
+                        const smx_detector_params_t* params, uint64_t seed, uint32_t frame,
+                        uint32_t pix_base, const uint32_t* frame_dev, double* E_out, void* adc_out,
+                        int adc_is_float32, int tile_side) {
+    return splat_detect_run(SMX_SPLAT_ARGS, gain_map, offset_map, params, seed, frame, pix_base,
+                            frame_dev, E_out, adc_out, adc_is_float32, tile_side, 3);
+}
+
+int smx_splat_detect_accumulate_ts(SMX_SPLAT_PARAMS, const double* gain_map,
+                                   const double* offset_map, const smx_detector_params_t* params,
+                                   uint64_t seed, uint32_t frame, uint32_t pix_base,
+                                   const uint32_t* frame_dev, double* E_out, void* adc_out,
+                                   int adc_is_float32, int tile_side) {
+    return splat_detect_run(SMX_SPLAT_ARGS, gain_map, offset_map, params, seed, frame, pix_base,
+                            frame_dev, E_out, adc_out, adc_is_float32, tile_side, 2);
+}
+
+// LUT mode (smx_lut_t): the same tile machinery on pixel-integral tables per
+// depth slice; key[i] is the slice of emitter i.
+#define SMX_LUT_PARAMS                                                                           \
+    int n, const double *u, const double *v, const double *w, const int *key, const int *fov_id, \
+        const smx_fov_t *fovs_dev, int nfov, int n_tiles, const float *lut_tables,               \
+        const smx_lut_t *lut, long long list_capacity, int accumulate, double *out,              \
+        void *workspace, size_t workspace_bytes, int *status_dev, hipStream_t stream
+
+int smx_splat_lut_ts(SMX_LUT_PARAMS, int stages, int tile_side) {
+    if (!lut || stages < 1 || stages > 3) {
+        smx::set_error("smx_splat_lut_ts: invalid arguments (stages=%d)", stages);
+        return SMX_EVALUE;
+    }
+    return splat_run(n, u, v, w, key, fov_id, fovs_dev, nfov, n_tiles, nullptr, 0, 0, 1, 2, 1e-9,
+                     1.0, list_capacity, accumulate, lut_tables, 1, lut->m, out, workspace,
+                     workspace_bytes, status_dev, stream, stages, tile_side, nullptr, lut);
+}
+
+int smx_splat_lut_detect_ts(SMX_LUT_PARAMS, const double* gain_map, const double* offset_map,
+                            const smx_detector_params_t* params, uint64_t seed, uint32_t frame,
+                            uint32_t pix_base, const uint32_t* frame_dev, double* E_out,
+                            void* adc_out, int adc_is_float32, int stages, int tile_side) {
+    if (!lut || stages < 1 || stages > 3 || !lut_tables || accumulate != 0 || !params ||
+        params->emgain <= 0 || !adc_out) {
+        smx::set_error("smx_splat_lut_detect_ts: needs tables, accumulate = 0, detector "
+                       "parameters with emgain > 0 and an ADC buffer");
+        return SMX_EVALUE;
+    }
+    DetectArgs d;
+    d.P = *params;
+    d.gain_map = gain_map;
+    d.offset_map = offset_map;
+    d.seed = seed;
+    d.frame = frame;
+    d.pix_base = pix_base;
+    d.frame_dev = frame_dev;
+    d.E_out = E_out;
+    d.adc_out = adc_out;
+    d.adc_f32 = adc_is_float32 ? 1 : 0;
+    return splat_run(n, u, v, w, key, fov_id, fovs_dev, nfov, n_tiles, nullptr, 0, 0, 1, 2, 1e-9,
+                     1.0, list_capacity, accumulate, lut_tables, 1, lut->m, out, workspace,
+                     workspace_bytes, status_dev, stream, stages, tile_side, &d, lut);
 }
 
 }  // extern "C"
