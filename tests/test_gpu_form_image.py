@@ -176,8 +176,10 @@ def test_reference_example_images(gpu):
     examples/twocolor.py (same calls, same RandomState(123)) reproduces their
     committed output images (tests/golden/*_u8.npz, from examples/*.png) on
     the background-only pixels: > 99% equal after the 8-bit rescale, all
-    within one grey level (the images predate the reference's current PSF
-    code, so pixels under emitters are not compared)."""
+    within one grey level.  (Pixels under emitters depend on the uniforms
+    and noise of each pixel as well as its expectation; their brightness is
+    pinned through the oracle, whose expectation this path equals to 1e-10,
+    in test_oracle_png.test_tirf_png_emitter_brightness.)"""
     import os
     import scopyon_amd
     from test_oracle_png import GOLDEN, _background_agreement
