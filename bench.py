@@ -284,11 +284,11 @@ def precision_leg(reps=10):
     return out
 
 
-def _replay_rate(sim, steps, warmup):
+def _replay_rate(sim, steps, warmup, graph_steps=1):
     import torch
-    sim.capture()
+    sim.capture(graph_steps)
     for _ in range(warmup):
-        sim.replay()
+        sim.replay(graph_steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.replay(steps)
@@ -360,65 +360,186 @@ def batched_uneven_leg(steps=20, warmup=3):
     sim = BatchedSimulator(_config(512, 0.033), [(int(s), int(s)) for s in sizes], counts,
                            D=0.1e-12, dt=0.033, seed=23)
     sec = _replay_rate(sim, steps, warmup)
+    # the same steps captured ten to a graph (one graph launch per 10 steps)
+    sec10 = _replay_rate(sim, steps, warmup, graph_steps=10)
     return {'workload': '256 FOVs, sizes from {128,256,512}^2, lognormal emitter counts '
                         '(median 5000) clipped to [100, 50000], hipGraph replay',
             'fovs': F, 'emitters': int(counts.sum()), 'pixels': int(sim.n_pix),
             'frames_per_sec': F / sec, 'ms_per_step': sec * 1e3,
+            'ms_per_step_10_steps_per_graph': sec10 * 1e3,
             'emitter_frames_per_sec': float(counts.sum()) / sec,
             'status_unsorted_lists': bool(sim.status() & 1),
             'tile_side': sim.tile_side,
             'tile_autotune_ms': {str(k): v for k, v in sim.tile_timings.items()}}
 
 
-def cpu_baseline(size, n_emit, sample_emitters, seed=1):
-    """Time the float64 oracle (single core) on a bounded sample of one frame:
-    `sample_emitters` emitters splatted + the full per-pixel EMCCD loop; the
-    splat time is scaled to `n_emit` emitters.  Also returns the GPU-vs-oracle
-    relative error of the noise-free image on the same sample."""
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def _oracle_frame_setup(size, n_emit, seed):
     import numpy
-    import torch
     from oracle import epifm as O, params as P
     ocfg = P.default_config()
     ocfg['magnification'] = 100.0
     ocfg['detector']['image_size'] = [size, size]
     ocfg['detector']['exposure_time'] = 0.033
+    ocfg['effects']['photo_bleaching']['switch'] = False
     p = P.epifm_params(ocfg, numpy.random.RandomState(0))
     rng = numpy.random.RandomState(seed)
-    pl = 16e-6 / 100.0
-    half = 0.5 * size * pl
-    xy = rng.uniform(-half, half, size=(sample_emitters, 2))
-    parts = numpy.column_stack([numpy.zeros(sample_emitters), xy, numpy.arange(sample_emitters),
-                                numpy.ones(sample_emitters)])
+    half = 0.5 * size * (16e-6 / 100.0)
+    xy = rng.uniform(-half, half, size=(n_emit, 2))
+    parts = numpy.column_stack([numpy.zeros(n_emit), xy, numpy.arange(n_emit), numpy.ones(n_emit)])
     psf = O.PSFCache(p)
-    psf.get(0.0)    # table build is a one-time cost, not per frame
+    psf.get(0.0)    # the table build is a one-time cost, not per frame
+    return p, psf, parts, rng
+
+
+def _pool_splat(args):
+    from oracle import epifm as O
+    p, table, parts, size = args
+    psf = O.PSFCache(p)
+    psf.tables[0] = table
+    import numpy
+    exp, _, _ = O.molecule_plane(p, psf, parts, (size, size), numpy.zeros(3), 0.033, {}, None,
+                                 None)
+    return exp
+
+
+def _pool_emccd(args):
+    import numpy
+    from oracle import epifm as O
+    E, emgain, seed = args
+    return O.emccd_signal(E, emgain, numpy.random.RandomState(seed), memo=False)
+
+
+def cpu_baseline_child(size, n_emit, seed=1, procs=None):
+    """CPU timings of one headline frame (no GPU is touched in this process):
+    1. the line-faithful float64 oracle on one core: all n_emit emitters
+       splatted (_epifm.py:1227-1333) and the per-pixel EMCCD loop over every
+       pixel (_epifm.py:402-416, pmf rebuilt per pixel as the reference does);
+    2. the same oracle fanned out over `procs` processes (emitter chunks, then
+       pixel chunks; the reference's processes > 1 Pool, _epifm.py:1239-1260,
+       404-407; not stream-equal, as the reference's Pool uses the global RNG);
+    3. a vectorised-numpy frame (oracle/vectorised.py: phase-table splat,
+       numpy Generator EMCCD), the best-effort single-core CPU form."""
+    import multiprocessing
+    import numpy
+    from oracle import epifm as O, vectorised as V
+    p, psf, parts, rng = _oracle_frame_setup(size, n_emit, seed)
     t0 = time.perf_counter()
     exp, _, _ = O.molecule_plane(p, psf, parts, (size, size), numpy.zeros(3), 0.033, {}, None,
                                  None)
     t_splat = time.perf_counter() - t0
-    cam = numpy.zeros((size, size, 2))
-    cam[:, :, 0] = exp
     t0 = time.perf_counter()
-    photons = cam[:, :, 0] + p.background_mean
-    E = p.QE * photons
+    E = p.QE * (exp + p.background_mean)
     noise = O.emccd_noise(E.shape, p.readout_noise, rng)
     sig = O.emccd_signal(E, p.emgain, rng, memo=False)
     O.adc_counts(sig + noise, p.adc_fullwell, p.adc_gain, p.adc_offset, p.adc_bit)
     t_det = time.perf_counter() - t0
-    per_frame = t_splat * (n_emit / sample_emitters) + t_det
-    # the batched path's noise-free image on the same particles
+    out = {'cpu_model': _cpu_model(), 'cpu_count': os.cpu_count(),
+           'single_core': {'seconds_per_frame': t_splat + t_det,
+                           'seconds': {'splat': t_splat, 'detector': t_det}}}
+    procs = procs or min(16, os.cpu_count() or 1)
+    table = psf.get(0.0)
+    chunks = numpy.array_split(parts, procs)
+    ctx = multiprocessing.get_context('fork')
+    with ctx.Pool(procs) as pool:
+        t0 = time.perf_counter()
+        parts_img = pool.map(_pool_splat, [(p, table, c, size) for c in chunks])
+        exp2 = numpy.sum(parts_img, axis=0)
+        t_s2 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        E2 = p.QE * (exp2 + p.background_mean)
+        noise2 = O.emccd_noise(E2.shape, p.readout_noise, rng)
+        rows = numpy.array_split(E2, procs)
+        sig2 = numpy.concatenate(pool.map(_pool_emccd, [(r, p.emgain, k) for k, r in
+                                                          enumerate(rows)]))
+        O.adc_counts(sig2 + noise2, p.adc_fullwell, p.adc_gain, p.adc_offset, p.adc_bit)
+        t_d2 = time.perf_counter() - t0
+    out['processes'] = {'procs': procs, 'seconds_per_frame': t_s2 + t_d2,
+                        'seconds': {'splat': t_s2, 'detector': t_d2}}
+    ps = V.PhaseSplat(psf.get(0.0), 16e-6 / 100.0)
+    w = numpy.full(n_emit, p_weight(p))
+    gen = numpy.random.Generator(numpy.random.Philox(seed))
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        img = ps.frame((size, size), parts[:, 1], parts[:, 2], w)
+        V.emccd_adc(img, p, gen)
+    t_v = (time.perf_counter() - t0) / reps
+    out['vectorised'] = {'seconds_per_frame': t_v,
+                         'max_rel_diff_vs_oracle': float(numpy.abs(img - exp).max() / exp.max())}
+    return out
+
+
+def p_weight(p):
+    """Emitter weight at the focal plane (_epifm.py:1280-1315), from the
+    oracle's parameters."""
+    import numpy
+    from oracle import params as P
+    amp, _ = P.snells_law(p.light_source)
+    n_emit = P.get_emit_photons(amp, 0.033, p.abs_coefficient, p.quantum_yield,
+                                p.fluorophore_radius)
+    return numpy.sum(p.fluoem_norm) * p.psf_normalization * (1.0 * n_emit / (4.0 * numpy.pi))
+
+
+def cpu_baseline(size, n_emit, timeout=400):
+    """Run cpu_baseline_child in a child process (so the CPU work never
+    shares a process with the GPU runtime) and shape its timings into the
+    benchmark's cpu_baseline record."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), '--cpu-baseline-child', '--size', str(size),
+           '--emitters', str(n_emit)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError('CPU baseline failed: ' + r.stderr[-2000:])
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    one = d['single_core']['seconds_per_frame']
+    return {'value': 1.0 / one, 'unit': 'frames/s', 'cores': 1, 'kind': 'port',
+            'sample': ('one full {0}x{0} frame, {1} emitters, z = 0: the float64 numpy/scipy '
+                       'oracle, single core, splat of every emitter + the per-pixel EMCCD pmf '
+                       'and inverse-CDF draw over all {2} pixels + ADC'.format(
+                           size, n_emit, size * size)),
+            'seconds': d['single_core']['seconds'],
+            'cpu_model': d['cpu_model'], 'cpu_count': d['cpu_count'],
+            'variants': {
+                'processes': {'value': 1.0 / d['processes']['seconds_per_frame'],
+                              'unit': 'frames/s', 'cores': d['processes']['procs'],
+                              'seconds': d['processes']['seconds'],
+                              'note': 'the oracle over a process pool (emitter chunks, then '
+                                      'pixel chunks), like the reference with processes > 1'},
+                'vectorised_numpy': {'value': 1.0 / d['vectorised']['seconds_per_frame'],
+                                     'unit': 'frames/s', 'cores': 1,
+                                     'max_rel_diff_vs_oracle':
+                                         d['vectorised']['max_rel_diff_vs_oracle'],
+                                     'note': 'phase-table splat + numpy Generator EMCCD '
+                                             '(statistically equivalent detector)'}}}
+
+
+def gpu_vs_oracle_error(size, sample_emitters=200, seed=1):
+    """Max relative error of the batched path's noise-free expectation
+    QE (photons + background) against the float64 oracle on a sample."""
+    import numpy
+    from oracle import epifm as O
     from scopyon_amd.batched import BatchedSimulator
+    p, psf, parts, _ = _oracle_frame_setup(size, sample_emitters, seed)
+    exp, _, _ = O.molecule_plane(p, psf, parts, (size, size), numpy.zeros(3), 0.033, {}, None,
+                                 None)
+    E = p.QE * (exp + p.background_mean)
     sim = BatchedSimulator(_config(size, 0.033), [(size, size)], sample_emitters, dt=0.033, seed=0)
     sim.set_positions(parts[:, 0], parts[:, 1], parts[:, 2])
     sim.form()
     got = sim.frame(0, 'photons').cpu().numpy()
     Eg = p.QE * (got + p.background_mean)
-    rel = float(numpy.max(numpy.abs(Eg - E) / numpy.abs(E)))
-    return {'value': 1.0 / per_frame, 'unit': 'frames/s', 'cores': 1, 'kind': 'port',
-            'sample': ('one {0}x{0} frame: {1} emitters splatted (scaled x{2:g} to {3}) + '
-                       'per-pixel EMCCD over all {4} pixels, float64 numpy/scipy oracle, '
-                       'single core'.format(size, sample_emitters, n_emit / sample_emitters,
-                                            n_emit, size * size)),
-            'seconds': {'splat_sample': t_splat, 'detector': t_det}}, rel
+    return float(numpy.max(numpy.abs(Eg - E) / numpy.abs(E)))
 
 
 def main():
@@ -430,8 +551,12 @@ def main():
     ap.add_argument('--emitters', type=int, default=10000)
     ap.add_argument('--size', type=int, default=512)
     ap.add_argument('--plain-steps', type=int, default=20)
-    ap.add_argument('--cpu-sample', type=int, default=200)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--legs', default='plain,api,uneven,scale3d,precision,fwd_bwd',
+                    help='comma list of the extra legs to run besides the headline batch '
+                         '(plain, api, uneven, scale3d, precision, fwd_bwd); "none" for only '
+                         'the headline (for profiling runs)')
+    ap.add_argument('--cpu-baseline-child', action='store_true', help=argparse.SUPPRESS)
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='only run a device copy of known size (for rocprofv3 --pmc scaling)')
     args = ap.parse_args()
@@ -442,6 +567,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.cpu_baseline_child:
+        print(json.dumps(cpu_baseline_child(args.size, args.emitters)))
         return
     import numpy
     import torch
@@ -464,29 +592,33 @@ def main():
     assert sim.status() & ~1 == 0
     batched_fps = F * args.steps / elapsed
 
-    # plain single-frame path: same kernels, one frame per step, eager launches
-    plain = BatchedSimulator(cfg, [(S, S)], args.emitters, D=0.1e-12, dt=dt, seed=99,
-                             boundary='reflect')
-    for _ in range(3):
-        plain.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
-        plain.step()
-    torch.cuda.synchronize()
-    plain_fps = args.plain_steps / (time.perf_counter() - t0)
-
-    # drop-in API: form_image with a RandomState (reference-exact EMCCD draws)
-    rng = numpy.random.RandomState(5)
-    pl = 16e-6 / 100.0
-    inputs = rng.uniform(-0.5 * S * pl, 0.5 * S * pl, size=(args.emitters, 2))
-    scopyon_amd.form_image(inputs, config=cfg, rng=rng)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    napi = 3
-    for _ in range(napi):
+    legs = set() if args.legs == 'none' else set(args.legs.split(','))
+    plain_fps = api_fps = None
+    if 'plain' in legs:
+        # plain single-frame path: same kernels, one frame per step, eager launches
+        plain = BatchedSimulator(cfg, [(S, S)], args.emitters, D=0.1e-12, dt=dt, seed=99,
+                                 boundary='reflect')
+        for _ in range(3):
+            plain.step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            plain.step()
+        torch.cuda.synchronize()
+        plain_fps = args.plain_steps / (time.perf_counter() - t0)
+        del plain
+    if 'api' in legs:
+        # drop-in API: form_image with a RandomState (reference-exact EMCCD draws)
+        rng = numpy.random.RandomState(5)
+        pl = 16e-6 / 100.0
+        inputs = rng.uniform(-0.5 * S * pl, 0.5 * S * pl, size=(args.emitters, 2))
         scopyon_amd.form_image(inputs, config=cfg, rng=rng)
-    api_fps = napi / (time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        napi = 3
+        for _ in range(napi):
+            scopyon_amd.form_image(inputs, config=cfg, rng=rng)
+        api_fps = napi / (time.perf_counter() - t0)
 
     # per-stage launch times (graph-replayed back-to-back launches, HIP events
     # on the launch stream) and the dominant kernel's roofline
@@ -522,10 +654,10 @@ def main():
                    'tile_autotune_ms': {str(k): v for k, v in sim.tile_timings.items()}},
         # plain = the same kernels, one frame per step, eager launches
         'plain_frames_per_sec': plain_fps,
-        'batched_over_plain': batched_fps / plain_fps,
+        'batched_over_plain': batched_fps / plain_fps if plain_fps else None,
         # drop-in form_image(): RandomState draws on the host, reference-exact EMCCD
         'api_form_image_frames_per_sec': api_fps,
-        'batched_over_api_form_image': batched_fps / api_fps,
+        'batched_over_api_form_image': batched_fps / api_fps if api_fps else None,
         'kernel_ms': kt,
         'kernel_ms_sum': sum(kt.values()),
         'roofline': {'bound': 'hbm', 'kernel': KERNELS.get(dom, dom), 'stage': dom,
@@ -538,15 +670,19 @@ def main():
                      'frac_implementation_bytes': impl / sec / 1e9 / HBM_PEAK_GBS,
                      'launch_ms': kt[dom]},
     }
-    result['batched_uneven'] = batched_uneven_leg()
-    result['scale_3d'] = scale3d_leg()
-    result['precision'] = precision_leg()
-    result['fwd_bwd'] = fwd_bwd()
-    result['fwd_bwd_per_sec'] = result['fwd_bwd']['fwd_bwd_per_sec']
+    if 'uneven' in legs:
+        result['batched_uneven'] = batched_uneven_leg()
+    if 'scale3d' in legs:
+        result['scale_3d'] = scale3d_leg()
+    if 'precision' in legs:
+        result['precision'] = precision_leg()
+    if 'fwd_bwd' in legs:
+        result['fwd_bwd'] = fwd_bwd()
+        result['fwd_bwd_per_sec'] = result['fwd_bwd']['fwd_bwd_per_sec']
+    result['max_rel_err_vs_fp64_oracle'] = gpu_vs_oracle_error(S)
     if not args.no_cpu:
-        base, rel = cpu_baseline(S, args.emitters, args.cpu_sample)
+        base = cpu_baseline(S, args.emitters)
         result['cpu_baseline'] = base
-        result['max_rel_err_vs_fp64_oracle'] = rel
         result['gpu_over_cpu'] = batched_fps / base['value']
     print(json.dumps(result))
 

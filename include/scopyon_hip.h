@@ -383,6 +383,18 @@ int smx_brownian_philox(int n, int ndim, double* x0, double* x1, double* x2, con
                         const double* upper, uint64_t seed, uint32_t step, uint32_t id_base,
                         const int* fov_id, int nfov, const double* fov_box,
                         const uint32_t* step_dev, hipStream_t stream);
+/* One Philox step with per-state, per-axis diffusion and state transitions
+ * (sampling2.py:33-68; per-axis D as in sampling.py:115-118):
+ * x_d += sqrt(2 D_tab[state][d] dt) N(0, 1) (device D_tab, nstates x ndim),
+ * boundary as smx_brownian_philox (lower/upper host), then, when pacc
+ * (device, nstates x nstates cumulative transition rows) is given,
+ * state = searchsorted(pacc[state], u, 'left') with u a Philox uniform of the
+ * particle.  state (device int32) may be NULL for one state. */
+int smx_brownian_philox_states(int n, int ndim, double* x0, double* x1, double* x2, int* state,
+                               int nstates, const double* D_tab, double dt, int boundary,
+                               const double* lower, const double* upper, const double* pacc,
+                               uint64_t seed, uint32_t step, uint32_t id_base,
+                               hipStream_t stream);
 /* counter[0] += by on the device (advances Philox counters inside graphs). */
 int smx_counter_add(uint32_t* counter, uint32_t by, hipStream_t stream);
 /* Uniform initial positions (sampling2.py:15-31) from Philox; lower/upper host. */

@@ -121,6 +121,7 @@ class EPIFMConfigs:
         self.ADConverter_fullwell = adc.fullwell
         self.ADConverter_fpn_type = adc.type
         self.ADConverter_fpn_count = adc.count
+        self.ADConverter_offset0 = adc.offset
         self.ADConverter_offset, self.ADConverter_gain = self._adc_gain(adc.offset, rng)
         eff = config.effects
         self.background_switch = eff.background.switch
@@ -161,9 +162,10 @@ class EPIFMConfigs:
         self.fluoex_norm, self.fluoem_norm = ex, em
         self.psf_wavelength = grid[i_em]
 
-    def _adc_gain(self, adc0, rng):
-        """_epifm.py:926-963 (fixed-pattern offsets drawn from rng)."""
-        nw, nh = self.detector_image_size
+    def _adc_gain(self, adc0, rng, shape=None):
+        """_epifm.py:926-963 (fixed-pattern offsets drawn from rng) for a
+        frame of `shape` (default: the configured image size)."""
+        nw, nh = self.detector_image_size if shape is None else shape
         kind, count = self.ADConverter_fpn_type, self.ADConverter_fpn_count
         if kind == 'none':
             offset = numpy.full(nw * nh, adc0)

@@ -97,6 +97,8 @@ SIGNATURES = {
                                       P, P, P]),
     'smx_brownian_philox': (c_int, [c_int, c_int, P, P, P, P, c_double, c_double, c_int, P, P,
                                     c_uint64, c_uint32, c_uint32, P, c_int, P, P, P]),
+    'smx_brownian_philox_states': (c_int, [c_int, c_int, P, P, P, P, c_int, P, c_double, c_int,
+                                           P, P, P, c_uint64, c_uint32, c_uint32, P]),
     'smx_counter_add': (c_int, [P, c_uint32, P]),
     'smx_uniform_init': (c_int, [c_int, c_int, P, P, P, P, P, c_uint64, c_uint32, c_uint32, P]),
 }

@@ -44,8 +44,6 @@ class BatchedSimulator:
         self.configs = c = EPIFMConfigs(conf, rng=numpy.random.RandomState(seed))
         if c.filter_error is not None:
             raise TypeError(c.filter_error)
-        if not c.uniform_adc():
-            raise NotImplementedError('fixed-pattern ADC maps are not supported in batches')
         if c.detector_type not in ('EMCCD', 'CCD', 'CMOS'):
             raise RuntimeError("Unknown detector type was given [{}]. ".format(c.detector_type)
                                + "Use either one of 'CMOS', 'CCD' or 'EMCCD'.")
@@ -89,6 +87,16 @@ class BatchedSimulator:
         self.slot_map = self.tables.slot_map()
         self.splatter = Splatter(c, self.tables, self.device)
         self.det = c.detector_params()
+        # fixed-pattern ADC offsets (_epifm.py:926-963): one map per FOV, drawn
+        # FOV after FOV at that FOV's size from one RandomState(seed)
+        self.gain_map = self.offset_map = None
+        if c.ADConverter_fpn_type != 'none':
+            fpn_rng = numpy.random.RandomState(self.seed)
+            maps = [c._adc_gain(c.ADConverter_offset0, fpn_rng, shape) for shape in self.fov_shapes]
+            self.offset_map = _device.upload(numpy.concatenate([o.ravel() for o, _ in maps]),
+                                             device=self.device)
+            self.gain_map = _device.upload(numpy.concatenate([g.ravel() for _, g in maps]),
+                                           device=self.device)
         self._table = None
         if c.detector_type == 'CMOS':
             t = numpy.loadtxt(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'catalog',
@@ -141,6 +149,7 @@ class BatchedSimulator:
             self.D = _device.upload(numpy.asarray(D, dtype=float), device=self.device)
             self.D_scalar = 0.0
         self.p_state = None
+        self._w_fresh = False
         # frame buffers and device counters
         self.photons = torch.zeros(self.n_pix, dtype=torch.float64, device=self.device)
         self.adc = torch.zeros(self.n_pix, dtype=adc_dtype, device=self.device)
@@ -180,6 +189,7 @@ class BatchedSimulator:
                       c.detector_type == 'EMCCD')
         self.keep_photons = bool(keep_photons) or not self.fused
         self.graph = None
+        self.graph_steps = 1
         self.tile_timings = {}
         if tile_side == 'auto':
             if self.use_phase or self.lut:
@@ -266,6 +276,7 @@ class BatchedSimulator:
                   self.splatter.wparams, self.dt, None, None, self.slot_map.data_ptr(),
                   self.slot_map.numel(), self.w.data_ptr(), self.key.data_ptr(), None,
                   _device.stream())
+        self._w_fresh = True
 
     def splat(self, entry='smx_splat_expected'):
         """Pixel integration of every FOV (entry: smx_splat_expected, or its two
@@ -313,7 +324,8 @@ class BatchedSimulator:
                       self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
                       self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.K.data_ptr(), t.lut,
                       self.capacity, 0, out, self.ws.data_ptr(), self.ws.numel(),
-                      self.splatter.status.data_ptr(), _device.stream(), None, None, self.det,
+                      self.splatter.status.data_ptr(), _device.stream(),
+                      _device.ptr(self.gain_map), _device.ptr(self.offset_map), self.det,
                       self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
                       self.adc.data_ptr(), f32, stages[entry], self.tile_side)
             return
@@ -322,14 +334,17 @@ class BatchedSimulator:
                   self.nfov, self.n_tiles, t.sat.data_ptr(), t.sat_elems, len(t.slot_of), t.ntab,
                   t.period, 1e-9, 1e-18, self.capacity, 0, _device.ptr(self.Ktab), self.k_format,
                   t.m, out, self.ws.data_ptr(), self.ws.numel(),
-                  self.splatter.status.data_ptr(), _device.stream(), None, None, self.det,
-                  self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
-                  self.adc.data_ptr(), f32, self.tile_side)
+                  self.splatter.status.data_ptr(), _device.stream(), _device.ptr(self.gain_map),
+                  _device.ptr(self.offset_map), self.det, self.seed, 0, 0,
+                  self.counter.data_ptr(), _device.ptr(self.E), self.adc.data_ptr(), f32,
+                  self.tile_side)
 
     def stages(self):
         """The launches of one step, in order, as callables (for timing)."""
-        st = {'brownian_philox': self.move, 'emitter_weights': self.weights,
-              'splat_bin': lambda: self.splat('smx_splat_bin')}
+        st = {'brownian_philox': self.move}
+        if self.ndim == 3:
+            st['emitter_weights'] = self.weights
+        st['splat_bin'] = lambda: self.splat('smx_splat_bin')
         if self.fused:
             st['splat_detect'] = lambda: self.splat_detect('smx_splat_detect_accumulate_ts')
         else:
@@ -342,49 +357,69 @@ class BatchedSimulator:
         """Detector + ADC of every pixel (EMCCD, or CCD / CMOS by configuration)."""
         f32 = 1 if self.adc.dtype == torch.float32 else 0
         if self.configs.detector_type == 'EMCCD':
-            _lib.call('smx_emccd_philox', self.n_pix, self.photons.data_ptr(), None, None,
-                      self.det, self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
+            _lib.call('smx_emccd_philox', self.n_pix, self.photons.data_ptr(),
+                      _device.ptr(self.gain_map), _device.ptr(self.offset_map), self.det,
+                      self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
                       self.adc.data_ptr(), f32, _device.stream())
             return
         tv, tc, tn = self._table if self._table is not None else (None, None, 0)
-        _lib.call('smx_photon_detector_philox', self.n_pix, self.photons.data_ptr(), None, None,
-                  self.det, _device.ptr(tv), _device.ptr(tc), tn, self.seed, 0, 0,
-                  self.counter.data_ptr(), _device.ptr(self.E), self.adc.data_ptr(), f32,
-                  _device.stream())
+        _lib.call('smx_photon_detector_philox', self.n_pix, self.photons.data_ptr(),
+                  _device.ptr(self.gain_map), _device.ptr(self.offset_map), self.det,
+                  _device.ptr(tv), _device.ptr(tc), tn, self.seed, 0, 0, self.counter.data_ptr(),
+                  _device.ptr(self.E), self.adc.data_ptr(), f32, _device.stream())
 
     def set_positions(self, depth, u, v):
         """Overwrite the emitter coordinates (concatenated over FOVs)."""
         for dst, src in ((self.depth, depth), (self.u, u), (self.v, v)):
             dst.copy_(torch.as_tensor(numpy.ascontiguousarray(src, dtype=numpy.float64)))
+        if self.ndim == 2:
+            self.weights()          # depth is not moved in 2D: weights stay current
+
+    def _step_weights(self):
+        # 2D batches keep every emitter at its depth and the exposure fixed, so
+        # the weights and table slots of the first step hold for every step
+        if self.ndim == 3 or not self._w_fresh:
+            self.weights()
 
     def step(self):
         """Advance every FOV by one time step and form + detect its frame."""
         self.move()
+        self._step_weights()
         if self.fused:
-            self.weights()
             self.splat_detect()
             self.advance()
         else:
-            self.form()
+            self.splat()
             self.detect()
 
     # ---- HIP graph ----
-    def capture(self):
-        """Capture one step in a HIP graph (static buffers, device counters)."""
+    def capture(self, steps=1):
+        """Capture `steps` consecutive steps in one HIP graph (static buffers,
+        device counters); replay() then runs them `steps` at a time."""
+        steps = int(steps)
+        if steps < 1:
+            raise ValueError('steps must be >= 1')
         self.step()          # warm: workspace, tables, code objects
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self.step()
+            for _ in range(steps):
+                self.step()
         self.graph = g
+        self.graph_steps = steps
         return g
 
     def replay(self, steps=1):
-        for _ in range(steps):
-            if self.graph is None:
+        """Advance `steps` time steps: graph replays when captured (steps must
+        be a multiple of the captured step count), else eager steps."""
+        if self.graph is None:
+            for _ in range(steps):
                 self.step()
-            else:
-                self.graph.replay()
+            return
+        if steps % self.graph_steps:
+            raise ValueError('replay({}) with a graph of {} steps'.format(steps, self.graph_steps))
+        for _ in range(steps // self.graph_steps):
+            self.graph.replay()
 
     # ---- outputs ----
     def frame(self, f, which='adc'):

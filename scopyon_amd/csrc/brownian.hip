@@ -120,6 +120,58 @@ __global__ void brownian_philox_kernel(int n, int ndim, double* __restrict__ x0,
     }
 }
 
+// One step with per-state, per-axis diffusion and optional state
+// transitions (sampling2.py:33-68 with a per-axis D as in sampling.py:115-118):
+// x_d += sqrt(2 D[s][d] dt) N(0, 1) from the same Philox blocks as
+// brownian_philox_kernel, the boundary, then s = searchsorted(pacc[s], u,
+// 'left') with u from block 2 of the particle's stream (transitions after the
+// move, as the reference orders them).
+__global__ void brownian_states_kernel(int n, int ndim, double* __restrict__ x0,
+                                       double* __restrict__ x1, double* __restrict__ x2,
+                                       int* __restrict__ state, int nstates,
+                                       const double* __restrict__ D_tab, double dt, int boundary,
+                                       double lo0, double lo1, double lo2, double hi0, double hi1,
+                                       double hi2, const double* __restrict__ pacc, uint64_t seed,
+                                       uint32_t step, uint32_t id_base) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int s = state ? state[i] : 0;
+    s = s < 0 ? 0 : (s >= nstates ? nstates - 1 : s);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u}, k0, k1);
+    double u1 = smx::u01_53_open0(r.x, r.y);
+    double u2 = smx::u01_53(r.z, r.w);
+    double rad = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(6.283185307179586 * u2, &sn, &cs);
+    double z[3] = {rad * cs, rad * sn, 0.0};
+    if (ndim > 2) {
+        smx::u32x4 q = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u}, k0, k1);
+        double v1 = smx::u01_53_open0(q.x, q.y);
+        double v2 = smx::u01_53(q.z, q.w);
+        z[2] = sqrt(-2.0 * log(v1)) * cos(6.283185307179586 * v2);
+    }
+    double* xs[3] = {x0, x1, x2};
+    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    for (int d = 0; d < ndim; ++d) {
+        double v = xs[d][i] + sqrt(2.0 * D_tab[(size_t)s * ndim + d] * dt) * z[d];
+        if (boundary == 1) {
+            v = (hi[d] > lo[d]) ? smx::np_remainder(v - lo[d], hi[d] - lo[d]) + lo[d] : lo[d];
+        } else if (boundary == 2) {
+            v = reflect(v, lo[d], hi[d]);
+        }
+        xs[d][i] = v;
+    }
+    if (pacc && state) {
+        smx::u32x4 t = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 2u}, k0, k1);
+        const double u = smx::u01_53(t.x, t.y);
+        const double* row = pacc + (size_t)s * nstates;
+        int k = 0;
+        while (k < nstates && row[k] < u) ++k;
+        state[i] = k < nstates ? k : nstates - 1;
+    }
+}
+
 __global__ void uniform_init_kernel(int n, int ndim, double* __restrict__ x0,
                                     double* __restrict__ x1, double* __restrict__ x2,
                                     double lo0, double lo1, double lo2, double hi0, double hi1,
@@ -181,6 +233,29 @@ int smx_brownian_philox(int n, int ndim, double* x0, double* x1, double* x2, con
                        ndim, x0, x1, x2, D, D_scalar, dt, boundary, lo[0], lo[1], lo[2], hi[0],
                        hi[1], hi[2], seed, step, id_base, fov_id, nfov, fov_box, step_dev);
     SMX_CHECK_LAUNCH("brownian_philox_kernel");
+    return SMX_OK;
+}
+
+int smx_brownian_philox_states(int n, int ndim, double* x0, double* x1, double* x2, int* state,
+                               int nstates, const double* D_tab, double dt, int boundary,
+                               const double* lower, const double* upper, const double* pacc,
+                               uint64_t seed, uint32_t step, uint32_t id_base,
+                               hipStream_t stream) {
+    if (n < 0 || ndim < 1 || ndim > 3 || nstates < 1 || boundary < 0 || boundary > 2 || !D_tab ||
+        (n > 0 && (!x0 || (ndim > 1 && !x1) || (ndim > 2 && !x2))) ||
+        (boundary && (!lower || !upper)) || (pacc && !state)) {
+        smx::set_error("smx_brownian_philox_states: invalid arguments (n=%d, ndim=%d, "
+                       "nstates=%d)", n, ndim, nstates);
+        return SMX_EVALUE;
+    }
+    if (n == 0) return SMX_OK;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (boundary)
+        for (int d = 0; d < ndim; ++d) { lo[d] = lower[d]; hi[d] = upper[d]; }
+    hipLaunchKernelGGL(brownian_states_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n,
+                       ndim, x0, x1, x2, state, nstates, D_tab, dt, boundary, lo[0], lo[1], lo[2],
+                       hi[0], hi[1], hi[2], pacc, seed, step, id_base);
+    SMX_CHECK_LAUNCH("brownian_states_kernel");
     return SMX_OK;
 }
 

@@ -133,7 +133,7 @@ def sample_inputs(t, *, N=None, conc=None, lower=None, upper=None, D=None, start
     T = len(moves)
     x0 = numpy.ascontiguousarray(points[:, :ndim])
     if isinstance(rng, PhiloxRNG):
-        traj = _philox_walk(rng, x0, numpy.tile(Dax, (n, 1)), moves, boundary=0)
+        traj, _ = _philox_walk(rng, x0, numpy.asarray(Dax, dtype=float)[None, :], moves)
     else:
         scale = numpy.empty((T, 1, ndim))
         xi = numpy.empty((T, n, ndim))
@@ -152,26 +152,42 @@ def sample_inputs(t, *, N=None, conc=None, lower=None, upper=None, D=None, start
     return out
 
 
-def _philox_walk(rng, x0, Dper, moves, boundary=0, lower=None, upper=None):
-    """Device Philox trajectory: x0 (n, ndim<=3), Dper (n, ndim) per-axis D."""
+def _philox_walk(rng, x0, D_tab, moves, boundary=0, lower=None, upper=None, state0=None,
+                 transmat=None):
+    """Device Philox trajectory: x0 (n, ndim <= 3), D_tab (nstates, ndim)
+    diffusion per state and axis, state0 (n,) initial states, transmat
+    (nstates, nstates) rates [1/s] (zero diagonal) for state transitions
+    after each move (sampling2.py:52-68).  Returns (trajectory (T+1, n, ndim),
+    states (T+1, n))."""
     dev = _device.require_gpu()
     n, ndim = x0.shape
     if ndim > 3:
         raise NotImplementedError('PhiloxRNG sampling supports ndim <= 3')
+    D_tab = numpy.asarray(D_tab, dtype=float).reshape(-1, ndim)
+    nstates = D_tab.shape[0]
     xs = [_device.upload(x0[:, d].copy(), device=dev) if d < ndim else
           torch.zeros(1, dtype=torch.float64, device=dev) for d in range(3)]
     traj = numpy.empty((len(moves) + 1, n, ndim))
     traj[0] = x0
+    s0 = numpy.zeros(n, dtype=numpy.int32) if state0 is None else numpy.asarray(state0, numpy.int32)
+    states = numpy.empty((len(moves) + 1, n), dtype=numpy.int32)
+    states[0] = s0
+    d_state = _device.upload(s0, dtype=torch.int32, device=dev)
     lo = _lib.host_doubles(lower[:ndim] if lower is not None else [], 3)
     hi = _lib.host_doubles(upper[:ndim] if upper is not None else [], 3)
-    same_axes = numpy.all(Dper == Dper[:, :1])
-    if not same_axes:
-        raise NotImplementedError('PhiloxRNG sampling needs the same D on every axis')
-    d_D = _device.upload(Dper[:, 0].copy(), device=dev)
+    d_D = _device.upload(numpy.ascontiguousarray(D_tab), device=dev)
     for k, dt in enumerate(moves):
-        _lib.call('smx_brownian_philox', n, ndim, xs[0].data_ptr(), xs[1].data_ptr(),
-                  xs[2].data_ptr(), d_D.data_ptr(), 0.0, float(dt), boundary, lo, hi, rng.seed,
-                  rng.next_counter(), 0, None, 0, None, None, _device.stream())
+        d_pacc = None
+        if transmat is not None:
+            P = 1 - numpy.exp(-transmat * dt)
+            assert (P.sum(axis=1) <= 1.0).all()
+            P.ravel()[::nstates + 1] = 1.0 - P.sum(axis=1)
+            d_pacc = _device.upload(numpy.ascontiguousarray(P.cumsum(axis=1)), device=dev)
+        _lib.call('smx_brownian_philox_states', n, ndim, xs[0].data_ptr(), xs[1].data_ptr(),
+                  xs[2].data_ptr(), d_state.data_ptr(), nstates, d_D.data_ptr(), float(dt),
+                  boundary, lo, hi, _device.ptr(d_pacc), rng.seed, rng.next_counter(), 0,
+                  _device.stream())
         for d in range(ndim):
             traj[k + 1, :, d] = xs[d].cpu().numpy()
-    return traj
+        states[k + 1] = d_state.cpu().numpy()
+    return traj, states

@@ -70,11 +70,10 @@ def sample(t, N, *, lower=None, upper=None, D=None, transmat=None, ndim=3, perio
     state0 = points[:, ndim].astype(numpy.int32)
     x0 = numpy.ascontiguousarray(points[:, :ndim])
     if isinstance(rng, PhiloxRNG):
-        if transmat is not None:
-            raise NotImplementedError('state transitions need a numpy RandomState')
-        Dper = numpy.repeat(D[state0][:, None], ndim, axis=1)
-        traj = _philox_walk(rng, x0, Dper, moves, boundary=mode, lower=lower, upper=upper)
-        states = numpy.repeat(state0[None, :], T + 1, axis=0)
+        # device draws: per-state D on every axis, transitions after each move
+        D_tab = numpy.repeat(numpy.asarray(D, dtype=float)[:, None], ndim, axis=1)
+        traj, states = _philox_walk(rng, x0, D_tab, moves, boundary=mode, lower=lower,
+                                    upper=upper, state0=state0, transmat=transmat)
     else:
         scale = numpy.empty((T, nstates, ndim))
         xi = numpy.empty((T, n, ndim))

@@ -195,6 +195,31 @@ def test_graph_replay_equals_eager(gpu):
     assert torch.equal(a.u, b.u) and torch.equal(a.v, b.v)
 
 
+def test_multi_step_graph_equals_eager(gpu):
+    """Three steps captured in one HIP graph, replayed twice, equal six eager
+    steps (ADC of the last step of each graph, positions)."""
+    from scopyon_amd.batched import BatchedSimulator
+    shapes = [(64, 64), (32, 96)]
+    a = BatchedSimulator(_cfg(), shapes, [200, 80], dt=0.033, seed=19)
+    b = BatchedSimulator(_cfg(), shapes, [200, 80], dt=0.033, seed=19)
+    outs = []
+    for k in range(7):
+        a.step()
+        if k in (0, 3, 6):
+            outs.append(a.adc.clone())
+    b.capture(steps=3)     # one eager warm-up step, then the 3-step graph
+    got = [b.adc.clone()]
+    for _ in range(2):
+        b.replay(3)
+        got.append(b.adc.clone())
+    torch.cuda.synchronize()
+    for x, y in zip(outs, got):
+        assert torch.equal(x, y)
+    assert torch.equal(a.u, b.u) and torch.equal(a.v, b.v)
+    with pytest.raises(ValueError):
+        b.replay(2)
+
+
 def test_reflecting_walls_hold(gpu):
     from scopyon_amd.batched import BatchedSimulator
     sim = BatchedSimulator(_cfg(), [(32, 32)], 500, D=5e-12, dt=0.033, seed=2)
