@@ -346,6 +346,16 @@ int smx_sep_gemm(int n, const float* x, const float* y, const float* z, const fl
                  const smx_gauss_params_t* params /* host */, int dtype, long long capacity,
                  float* mu, void* workspace, size_t workspace_bytes, int* status_dev,
                  hipStream_t stream);
+/* smx_sep_gemm with the Poisson NLL fused into the tile pass: loss[f] = sum
+ * over frame f of mu - k log(mu) (float64 terms as smx_gauss_poisson_nll,
+ * summed per tile, then over the frame's tiles in order) and grad_mu (may be
+ * NULL) = 1 - k / mu, from the tile's accumulators; mu (may be NULL) is
+ * written only when given.  Same workspace as smx_sep_gemm. */
+int smx_sep_gemm_nll(int n, const float* x, const float* y, const float* z, const float* I,
+                     const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                     const smx_gauss_params_t* params /* host */, int dtype, long long capacity,
+                     float* mu, const float* k, float* grad_mu, double* loss, void* workspace,
+                     size_t workspace_bytes, int* status_dev, hipStream_t stream);
 /* loss[f] = sum over frame f of mu - k log(mu) (float64); grad_mu (may be NULL)
  * = 1 - k / mu.  workspace: smx_gauss_nll_workspace_bytes(nframes, max pixels
  * of any frame). */
@@ -356,7 +366,10 @@ int smx_gauss_poisson_nll(const float* mu, const float* k, const smx_fov_t* fram
 /* Hand-written backward of smx_gauss_forward: given grad_mu = dL/dmu, writes
  * dL/dx, dL/dy, dL/dz, dL/dI per emitter (any output may be NULL) and, when
  * gsigma (device scalar) is not NULL, dL/dsigma summed over all emitters in a
- * fixed order (float64); workspace then holds at least 8 * (n + 256) bytes. */
+ * fixed order (float64).  The window factors G, dG/dx, dG/dsigma come from a
+ * table by sub-pixel phase built in float64 for params->sigma (1025 phases,
+ * linear interpolation).  workspace: smx_gauss_backward_workspace_bytes(n). */
+size_t smx_gauss_backward_workspace_bytes(int n);
 int smx_gauss_backward(int n, const float* x, const float* y, const float* z, const float* I,
                        const int* frame_id, const smx_fov_t* frames_dev, int nframes,
                        const smx_gauss_params_t* params /* host */, const float* grad_mu,

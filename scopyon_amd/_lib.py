@@ -117,6 +117,10 @@ SIGNATURES.update({
                                   ctypes.POINTER(GaussParamsT), c_longlong, P, P, c_size_t, P, P]),
     'smx_sep_gemm': (c_int, [c_int, P, P, P, P, P, P, c_int, c_int, ctypes.POINTER(GaussParamsT),
                              c_int, c_longlong, P, P, c_size_t, P, P]),
+    'smx_sep_gemm_nll': (c_int, [c_int, P, P, P, P, P, P, c_int, c_int,
+                                 ctypes.POINTER(GaussParamsT), c_int, c_longlong, P, P, P, P, P,
+                                 c_size_t, P, P]),
+    'smx_gauss_backward_workspace_bytes': (c_size_t, [c_int]),
     'smx_gauss_nll_workspace_bytes': (c_size_t, [c_int, c_int]),
     'smx_gauss_poisson_nll': (c_int, [P, P, P, c_int, c_int, P, P, P, c_size_t, P]),
     'smx_gauss_backward': (c_int, [c_int, P, P, P, P, P, P, c_int, ctypes.POINTER(GaussParamsT),

@@ -57,13 +57,49 @@ __device__ inline float g_pixel_edges(float t0, float t1, float e0, float e1) {
     return 0.5f * (erff(t1) - erff(t0));
 }
 
-// Normal density N(t) = exp(-t^2 / (2 sigma^2)) / (sqrt(2 pi) sigma).
-__device__ inline float normal_pdf(float t, float inv_sigma) {
-    float u = t * inv_sigma;
-    return INV_SQRT2PI * inv_sigma * expf(-0.5f * u * u);
+__device__ inline bool finite_coord(float v) { return v > -1.0e6f && v < 1.0e9f; }
+
+// Window factors by sub-pixel phase.  For an emitter at x = floor(x) + d,
+// window pixel b (of 2R + 1) spans [b - R - d, b - R + 1 - d) relative to x,
+// so its factors depend on (d, b) only: G (the pixel-integrated unit
+// Gaussian), dG/dx and dG/dsigma.  GQ + 1 phases d = q / GQ are evaluated in
+// float64 once per sigma (gauss_table_kernel) and read with linear
+// interpolation in d (error ~1e-8 of the peak factor at GQ = 1024 and sigma
+// >= 0.5 px: the second derivative in d is bounded by 0.24 / sigma^2).
+constexpr int GQ = 1024;
+constexpr int GB = 16;            // factor slots per phase (window <= 15)
+constexpr int TABLE_ENTRIES = (GQ + 1) * GB;
+
+__global__ void gauss_table_kernel(smx_gauss_params_t P, float4* __restrict__ tab) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= TABLE_ENTRIES) return;
+    const int q = idx / GB, b = idx % GB;
+    const int R = P.radius, W = 2 * R + 1;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (b < W) {
+        const double sigma = (double)P.sigma;
+        const double e0 = (double)(b - R) - (double)q / GQ, e1 = e0 + 1.0;
+        const double s = 0.70710678118654752 / sigma;
+        double g;
+        if (e0 >= 0.0) g = 0.5 * (erfc(e0 * s) - erfc(e1 * s));
+        else if (e1 <= 0.0) g = 0.5 * (erfc(-e1 * s) - erfc(-e0 * s));
+        else g = 0.5 * (erf(e1 * s) - erf(e0 * s));
+        const double c = 0.39894228040143268 / sigma;
+        const double p0 = c * exp(-0.5 * (e0 / sigma) * (e0 / sigma));
+        const double p1 = c * exp(-0.5 * (e1 / sigma) * (e1 / sigma));
+        v = make_float4((float)g, (float)(p0 - p1), (float)(-(e1 * p1 - e0 * p0) / sigma), 0.0f);
+    }
+    tab[idx] = v;
 }
 
-__device__ inline bool finite_coord(float v) { return v > -1.0e6f && v < 1.0e9f; }
+// (G, dG/dx, dG/dsigma) of window pixel b at sub-pixel offset d in [0, 1).
+__device__ inline float4 table_factor(const float4* __restrict__ tab, float d, int b) {
+    const float t = d * (float)GQ;
+    const int q = min((int)t, GQ - 1);
+    const float f = t - (float)q;
+    const float4 a = tab[q * GB + b], c = tab[(q + 1) * GB + b];
+    return make_float4(a.x + f * (c.x - a.x), a.y + f * (c.y - a.y), a.z + f * (c.z - a.z), 0.0f);
+}
 
 struct TileRange { int r0, r1, c0, c1; };
 
@@ -380,6 +416,7 @@ struct BwdArgs {
     float* gz;
     float* gI;
     double* gsig_part;   // per emitter: dL/dsigma contribution (may be NULL)
+    const float4* table; // window factors by phase (gauss_table_kernel)
 };
 
 __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
@@ -387,7 +424,6 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
     const int e = blockIdx.x * 16 + (threadIdx.x >> 4);
     const int j = threadIdx.x & 15;
     const int R = A.P.radius, W = 2 * R + 1;
-    const float sigma = A.P.sigma, inv_sigma = 1.0f / sigma, s = INV_SQRT2 * inv_sigma;
     bool valid = e < A.n;
     const int f = valid ? frame_of(A.frame_id, e) : 0;
     valid = valid && f >= 0 && f < A.nframes;
@@ -404,31 +440,18 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
         valid = finite_coord(xe) && finite_coord(ye);
     }
     const int lu = valid ? (int)floorf(xe) - R : 0, lv = valid ? (int)floorf(ye) - R : 0;
-    // window edge j (lane j, j = 0..W) along both axes: offset from the
-    // emitter, erfc(|t|) and the normal density; pixel j combines edges j and
-    // j + 1 (the next lane of the group), so every lane evaluates one erfc and
-    // one exp per axis instead of two erf-type pairs per pixel
-    float ax = 0.0f, bx = 0.0f, eax = 0.0f, ebx = 0.0f, nax = 0.0f, nbx = 0.0f;
-    if (valid && j <= W) {
-        ax = (float)(lu + j) - xe;
-        bx = (float)(lv + j) - ye;
-        eax = erfcf(fabsf(ax * s));
-        ebx = erfcf(fabsf(bx * s));
-        nax = normal_pdf(ax, inv_sigma);
-        nbx = normal_pdf(bx, inv_sigma);
-    }
-    const float ax1 = __shfl_down(ax, 1, 16), bx1 = __shfl_down(bx, 1, 16);
-    const float eax1 = __shfl_down(eax, 1, 16), ebx1 = __shfl_down(ebx, 1, 16);
-    const float nax1 = __shfl_down(nax, 1, 16), nbx1 = __shfl_down(nbx, 1, 16);
-    // this lane's window row factors (row lu + j) and column factors (col lv + j)
+    // this lane's window row factors (row lu + j) and column factors (col lv + j):
+    // G, dG/dx and dG/dsigma from the phase table
     float gxr = 0.0f, dgxr = 0.0f, sgxr = 0.0f, gyc = 0.0f, dgyc = 0.0f, sgyc = 0.0f;
     if (valid && j < W) {
-        gxr = g_pixel_edges(ax * s, ax1 * s, eax, eax1);
-        gyc = g_pixel_edges(bx * s, bx1 * s, ebx, ebx1);
-        dgxr = nax - nax1;                                  // dG/dx
-        dgyc = nbx - nbx1;
-        sgxr = -(ax1 * nax1 - ax * nax) * inv_sigma;        // dG/dsigma
-        sgyc = -(bx1 * nbx1 - bx * nbx) * inv_sigma;
+        const float4 fx = table_factor(A.table, xe - floorf(xe), j);
+        const float4 fy = table_factor(A.table, ye - floorf(ye), j);
+        gxr = fx.x;
+        dgxr = fx.y;
+        sgxr = fx.z;
+        gyc = fy.x;
+        dgyc = fy.y;
+        sgyc = fy.z;
     }
     // the group's row factors go through LDS: one 16-byte broadcast read per
     // row instead of three cross-lane shuffles
@@ -500,35 +523,26 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// Per emitter: the window origin and the (2R+1)-pixel factors along each axis,
-// from one tail mass per pixel edge, q(b) = erfc(|b - x| / (sqrt(2) sigma)) / 2:
-// G = q(i) - q(i+1) right of x, q(i+1) - q(i) left of x, 1 - q(i) - q(i+1)
-// across x.  32 lanes per emitter: lane t handles edge t % 16 of axis t / 16.
+// Per emitter: the window origin and the (2R+1)-pixel factors along each
+// axis from the phase table (gauss_table_kernel).  32 lanes per emitter: lane
+// t handles window pixel t % 16 of axis t / 16; a block of 256 threads takes
+// 8 emitters at a time, striding over the launch.
 __global__ void __launch_bounds__(256) sep_factor_kernel(int n, const float* __restrict__ x,
                                                          const float* __restrict__ y,
                                                          const float* __restrict__ z,
                                                          const float* __restrict__ I,
                                                          smx_gauss_params_t P,
+                                                         const float4* __restrict__ tab,
                                                          float* __restrict__ fac,
                                                          int2* __restrict__ lo_out) {
-    const int e = blockIdx.x * 8 + (threadIdx.x >> 5);
     const int t = threadIdx.x & 31, ax = t >> 4, b = t & 15;
     const int R = P.radius, W = 2 * R + 1;
-    const float s = INV_SQRT2 / P.sigma;
-    const bool valid = e < n;
-    const float pos = valid ? (ax ? y[e] : x[e]) : 0.0f;
-    const bool fin = finite_coord(pos);
-    const int lo = fin ? (int)floorf(pos) - R : 0;
-    const float tb = ((float)(lo + b) - pos) * s;
-    const float q = 0.5f * erfcf(fabsf(tb));
-    const float qn = __shfl_down(q, 1, 16);          // edge b + 1 (same axis)
-    const float edge = (float)(lo + b);
-    float g;
-    if (edge >= pos) g = q - qn;                      // pixel right of x
-    else if (edge + 1.0f <= pos) g = qn - q;          // pixel left of x
-    else g = 1.0f - q - qn;                           // pixel containing x
-    if (b >= W || !fin) g = 0.0f;
-    if (valid) {
+    for (int e = blockIdx.x * 8 + (threadIdx.x >> 5); e < n; e += gridDim.x * 8) {
+        const float pos = ax ? y[e] : x[e];
+        const bool fin = finite_coord(pos);
+        const float fl = floorf(pos);
+        float g = 0.0f;
+        if (fin && b < W) g = table_factor(tab, pos - fl, b).x;
         float amp = 1.0f;
         if (ax == 0) amp = I[e] * (z ? expf(-fabsf(z[e]) * P.inv_depth) : 1.0f);
         fac[(size_t)e * 32 + t] = ax == 0 ? amp * g : g;
@@ -551,13 +565,20 @@ struct SepArgs {
     long long capacity;
     int W;
     float background;
-    float* mu;
+    float* mu;           // may be NULL when the NLL is fused (NLL)
     int* status;
+    const float* k;      // NLL: observed counts per pixel
+    float* grad_mu;      // NLL: dL/dmu = 1 - k / mu (may be NULL)
+    double* tile_nll;    // NLL: sum over the tile of mu - k log(mu)
 };
 
 template <typename T> struct Frag8 { typedef T type __attribute__((ext_vector_type(8))); };
 
-template <int DT>   // 0: float32 (32x32x2), 1: float16 (32x32x16), 2: bfloat16 (32x32x16)
+// DT: operand type, 0: float32 (32x32x2), 1: float16 (32x32x16), 2: bfloat16
+// (32x32x16).  NLL: the Poisson NLL of the tile (float64 log, the formula of
+// gauss_nll_kernel) and its image gradient in the epilogue, from the
+// accumulators, instead of a pass over the written image.
+template <int DT, bool NLL>
 __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
     __shared__ int s_sort[PSORT_CAP];
     __shared__ float s_fac[16][33];
@@ -668,13 +689,42 @@ __global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
     }
     // acc[i] holds row 8 (i / 4) + 4 half + i % 4, column m of the tile
     const int pj = tc * TS + m;
+    double part = 0.0;
     if (pj < F.nh) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const int pi = tr * TS + 8 * (i >> 2) + 4 * half + (i & 3);
-            if (pi < F.nw) A.mu[(size_t)F.pix_off + (size_t)pi * F.nh + pj] = A.background + acc[i];
+            if (pi >= F.nw) continue;
+            const size_t o = (size_t)F.pix_off + (size_t)pi * F.nh + pj;
+            const float mv = A.background + acc[i];
+            if (A.mu) A.mu[o] = mv;
+            if (NLL) {
+                const float kk = A.k[o];
+                const float mc = mv > 0.0f ? mv : 1.0e-30f;
+                const double md = (double)mc;
+                part += md - (double)kk * log(md);
+                if (A.grad_mu) A.grad_mu[o] = 1.0f - kk / mc;
+            }
         }
     }
+    if (NLL) {
+        // fixed reduction tree over the wave: the tile sum is deterministic
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+        if (l == 0) A.tile_nll[tile] = part;
+    }
+}
+
+// loss[f] = sum of frame f's tile sums in tile order.
+__global__ void gauss_tile_nll_finish_kernel(const smx_fov_t* __restrict__ frames, int nframes,
+                                             const double* __restrict__ tile_nll,
+                                             double* __restrict__ loss) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nframes) return;
+    const smx_fov_t F = frames[f];
+    double s = 0.0;
+    for (int t = 0; t < F.tiles_r * F.tiles_c; ++t) s += tile_nll[F.tile_off + t];
+    loss[f] = s;
 }
 
 struct GaussWorkspace {
@@ -686,6 +736,8 @@ struct GaussWorkspace {
     double* gsig;      // n
     float* fac;        // n x 32: amp * G along rows (16), G along columns (16)
     int2* lo;          // n: window origin
+    float4* table;     // TABLE_ENTRIES window factors by phase
+    double* tile_nll;  // n_tiles: fused NLL per tile
     void* cub_tmp;
     size_t cub_bytes;
 };
@@ -701,6 +753,8 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     size_t o_sig = take(sizeof(double) * (size_t)(n > 0 ? n : 1));
     size_t o_fac = take(sizeof(float) * 32 * (size_t)(n > 0 ? n : 1));
     size_t o_lo = take(sizeof(int2) * (size_t)(n > 0 ? n : 1));
+    size_t o_tab = take(sizeof(float4) * (size_t)TABLE_ENTRIES);
+    size_t o_tnll = take(sizeof(double) * (size_t)n_tiles);
     size_t cb = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cb, (int*)nullptr, (int*)nullptr, n_tiles + 1);
     size_t o_cub = take(cb);
@@ -713,6 +767,8 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
         ws->gsig = (double*)(base + o_sig);
         ws->fac = (float*)(base + o_fac);
         ws->lo = (int2*)(base + o_lo);
+        ws->table = (float4*)(base + o_tab);
+        ws->tile_nll = (double*)(base + o_tnll);
         ws->cub_tmp = base + o_cub;
         ws->cub_bytes = cb;
     }
@@ -726,6 +782,12 @@ bool params_ok(const smx_gauss_params_t* P) {
 
 constexpr int NLL_PER = 4096;
 
+// backward workspace: per-emitter dL/dsigma terms and their partial sums
+// (float64), then the phase table
+size_t backward_table_offset(int n) {
+    return align256(sizeof(double) * ((size_t)(n > 0 ? n : 1) + SUM_BLOCKS));
+}
+
 }  // namespace
 
 extern "C" {
@@ -733,6 +795,11 @@ extern "C" {
 size_t smx_gauss_workspace_bytes(int n, int n_tiles, long long capacity) {
     if (n < 0 || n_tiles <= 0 || capacity < 0) return 0;
     return gauss_layout(n, n_tiles, capacity, nullptr, nullptr);
+}
+
+size_t smx_gauss_backward_workspace_bytes(int n) {
+    if (n < 0) return 0;
+    return backward_table_offset(n) + sizeof(float4) * (size_t)TABLE_ENTRIES;
 }
 
 }  // extern "C"
@@ -762,14 +829,21 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
                                ws.tile_count, cs);
         SMX_CHECK_LAUNCH("gauss_bin_count_kernel");
     }
-    size_t cb = ws.cub_bytes;
-    hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                       stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
-    SMX_CHECK_LAUNCH("compact_counts_kernel");
-    if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
-                                                        ws.tile_begin, n_tiles + 1, stream),
-                       "hipcub ExclusiveSum"))
-        return SMX_ERUNTIME;
+    if (n_tiles + 1 <= smx::SCAN_MAX) {
+        // one workgroup scans the counters in place of compact + device scan
+        hipLaunchKernelGGL(smx::scan_counts_kernel, dim3(1), dim3(smx::SCAN_THREADS), 0, stream,
+                           ws.tile_count, cs, n_tiles + 1, ws.tile_begin);
+        SMX_CHECK_LAUNCH("scan_counts_kernel");
+    } else {
+        size_t cb = ws.cub_bytes;
+        hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256),
+                           0, stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
+        SMX_CHECK_LAUNCH("compact_counts_kernel");
+        if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
+                                                            ws.tile_begin, n_tiles + 1, stream),
+                           "hipcub ExclusiveSum"))
+            return SMX_ERUNTIME;
+    }
     if (n > 0) {
         if (ept == 4)
             hipLaunchKernelGGL(gauss_bin_fill_kernel<4>, dim3(nbb), dim3(smx::BIN_THREADS), 0,
@@ -830,25 +904,33 @@ int smx_gauss_forward(int n, const float* x, const float* y, const float* z, con
     return SMX_OK;
 }
 
-int smx_sep_gemm(int n, const float* x, const float* y, const float* z, const float* I,
+}  // extern "C"
+
+namespace {
+
+// Binning, phase table, factors and the matrix-core tile pass (with the NLL
+// epilogue when k is given).
+int sep_gemm_run(int n, const float* x, const float* y, const float* z, const float* I,
                  const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
                  const smx_gauss_params_t* params, int dtype, long long capacity, float* mu,
-                 void* workspace, size_t workspace_bytes, int* status_dev, hipStream_t stream) {
-    int rc = forward_checks("smx_sep_gemm", n, x, y, z, I, frames_dev, nframes, n_tiles, params,
-                            capacity, mu, workspace, workspace_bytes, status_dev, false);
-    if (rc) return rc;
+                 const float* k, float* grad_mu, double* loss, void* workspace,
+                 size_t workspace_bytes, int* status_dev, hipStream_t stream) {
     if (dtype < 0 || dtype > 2) {
         smx::set_error("smx_sep_gemm: dtype must be 0 (float32), 1 (float16) or 2 (bfloat16)");
         return SMX_EVALUE;
     }
     GaussWorkspace ws;
     gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
-    rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, params->radius, capacity, ws,
-                   status_dev, stream);
+    int rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, params->radius, capacity,
+                       ws, status_dev, stream);
     if (rc) return rc;
     if (n > 0) {
-        hipLaunchKernelGGL(sep_factor_kernel, dim3((n + 7) / 8), dim3(256), 0, stream, n, x, y, z,
-                           I, *params, ws.fac, ws.lo);
+        hipLaunchKernelGGL(gauss_table_kernel, dim3((TABLE_ENTRIES + 255) / 256), dim3(256), 0,
+                           stream, *params, ws.table);
+        SMX_CHECK_LAUNCH("gauss_table_kernel");
+        const int blocks = (int)min(((long long)n + 7) / 8, 4096LL);
+        hipLaunchKernelGGL(sep_factor_kernel, dim3(blocks), dim3(256), 0, stream, n, x, y, z, I,
+                           *params, ws.table, ws.fac, ws.lo);
         SMX_CHECK_LAUNCH("sep_factor_kernel");
     }
     SepArgs A;
@@ -856,11 +938,57 @@ int smx_sep_gemm(int n, const float* x, const float* y, const float* z, const fl
     A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
     A.capacity = capacity; A.W = 2 * params->radius + 1; A.background = params->background;
     A.mu = mu; A.status = status_dev;
-    if (dtype == 0) hipLaunchKernelGGL(sep_gemm_kernel<0>, dim3(n_tiles), dim3(64), 0, stream, A);
-    else if (dtype == 1) hipLaunchKernelGGL(sep_gemm_kernel<1>, dim3(n_tiles), dim3(64), 0, stream, A);
-    else hipLaunchKernelGGL(sep_gemm_kernel<2>, dim3(n_tiles), dim3(64), 0, stream, A);
+    A.k = k; A.grad_mu = grad_mu; A.tile_nll = ws.tile_nll;
+    auto launch = [&](auto nll) {
+        constexpr bool L = decltype(nll)::value;
+        if (dtype == 0) hipLaunchKernelGGL((sep_gemm_kernel<0, L>), dim3(n_tiles), dim3(64), 0, stream, A);
+        else if (dtype == 1) hipLaunchKernelGGL((sep_gemm_kernel<1, L>), dim3(n_tiles), dim3(64), 0, stream, A);
+        else hipLaunchKernelGGL((sep_gemm_kernel<2, L>), dim3(n_tiles), dim3(64), 0, stream, A);
+    };
+    if (k) launch(std::true_type());
+    else launch(std::false_type());
     SMX_CHECK_LAUNCH("sep_gemm_kernel");
+    if (k) {
+        hipLaunchKernelGGL(gauss_tile_nll_finish_kernel, dim3((nframes + 63) / 64), dim3(64), 0,
+                           stream, frames_dev, nframes, ws.tile_nll, loss);
+        SMX_CHECK_LAUNCH("gauss_tile_nll_finish_kernel");
+    }
     return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smx_sep_gemm(int n, const float* x, const float* y, const float* z, const float* I,
+                 const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                 const smx_gauss_params_t* params, int dtype, long long capacity, float* mu,
+                 void* workspace, size_t workspace_bytes, int* status_dev, hipStream_t stream) {
+    int rc = forward_checks("smx_sep_gemm", n, x, y, z, I, frames_dev, nframes, n_tiles, params,
+                            capacity, mu, workspace, workspace_bytes, status_dev, false);
+    if (rc) return rc;
+    return sep_gemm_run(n, x, y, z, I, frame_id, frames_dev, nframes, n_tiles, params, dtype,
+                        capacity, mu, nullptr, nullptr, nullptr, workspace, workspace_bytes,
+                        status_dev, stream);
+}
+
+int smx_sep_gemm_nll(int n, const float* x, const float* y, const float* z, const float* I,
+                     const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,
+                     const smx_gauss_params_t* params, int dtype, long long capacity, float* mu,
+                     const float* k, float* grad_mu, double* loss, void* workspace,
+                     size_t workspace_bytes, int* status_dev, hipStream_t stream) {
+    // mu may be NULL here: the loss and its gradient need no image in memory
+    int rc = forward_checks("smx_sep_gemm_nll", n, x, y, z, I, frames_dev, nframes, n_tiles,
+                            params, capacity, mu ? mu : (const float*)k, workspace,
+                            workspace_bytes, status_dev, false);
+    if (rc) return rc;
+    if (!k || !loss) {
+        smx::set_error("smx_sep_gemm_nll: counts k and the loss buffer are required");
+        return SMX_EVALUE;
+    }
+    return sep_gemm_run(n, x, y, z, I, frame_id, frames_dev, nframes, n_tiles, params, dtype,
+                        capacity, mu, k, grad_mu, loss, workspace, workspace_bytes, status_dev,
+                        stream);
 }
 
 size_t smx_gauss_nll_workspace_bytes(int nframes, int max_frame_pixels) {
@@ -893,12 +1021,13 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
                        float* gy, float* gz, float* gI, double* gsigma, void* workspace,
                        size_t workspace_bytes, hipStream_t stream) {
     if (n < 0 || nframes <= 0 || !frames_dev || !grad_mu || !params_ok(params) ||
-        (n > 0 && (!x || !y || !z || !I)) ||
-        (gsigma && (!workspace ||
-                    workspace_bytes < sizeof(double) * ((size_t)(n > 0 ? n : 1) + SUM_BLOCKS)))) {
-        smx::set_error("smx_gauss_backward: invalid arguments (n=%d, nframes=%d)", n, nframes);
+        (n > 0 && (!x || !y || !z || !I)) || !workspace ||
+        workspace_bytes < smx_gauss_backward_workspace_bytes(n)) {
+        smx::set_error("smx_gauss_backward: invalid arguments (n=%d, nframes=%d, workspace %zu "
+                       "bytes)", n, nframes, workspace_bytes);
         return SMX_EVALUE;
     }
+    float4* table = (float4*)((char*)workspace + backward_table_offset(n));
     if (n == 0) {
         if (gsigma) hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, stream,
                                        (const double*)workspace, 0, 1, gsigma);
@@ -908,6 +1037,10 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
     B.n = n; B.x = x; B.y = y; B.z = z; B.I = I; B.frame_id = frame_id; B.frames = frames_dev;
     B.nframes = nframes; B.P = *params; B.grad_mu = grad_mu; B.gx = gx; B.gy = gy; B.gz = gz;
     B.gI = gI; B.gsig_part = gsigma ? (double*)workspace : nullptr;
+    B.table = table;
+    hipLaunchKernelGGL(gauss_table_kernel, dim3((TABLE_ENTRIES + 255) / 256), dim3(256), 0, stream,
+                       *params, table);
+    SMX_CHECK_LAUNCH("gauss_table_kernel");
     hipLaunchKernelGGL(gauss_backward_kernel, dim3((n + 15) / 16), dim3(256), 0, stream, B);
     SMX_CHECK_LAUNCH("gauss_backward_kernel");
     if (gsigma) {

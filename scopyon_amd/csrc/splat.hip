@@ -1350,14 +1350,21 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                 SMX_CHECK_LAUNCH("bin_count_kernel");
             }
         }
-        size_t cb = ws.cub_bytes;
-        hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                           stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
-        SMX_CHECK_LAUNCH("compact_counts_kernel");
-        if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
-                                                            ws.tile_begin, n_tiles + 1, stream),
-                           "hipcub ExclusiveSum"))
-            return SMX_ERUNTIME;
+        if (n_tiles + 1 <= smx::SCAN_MAX) {
+            // one workgroup scans the counters in place of compact + device scan
+            hipLaunchKernelGGL(smx::scan_counts_kernel, dim3(1), dim3(smx::SCAN_THREADS), 0, stream,
+                               ws.tile_count, cs, n_tiles + 1, ws.tile_begin);
+            SMX_CHECK_LAUNCH("scan_counts_kernel");
+        } else {
+            size_t cb = ws.cub_bytes;
+            hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256),
+                               0, stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
+            SMX_CHECK_LAUNCH("compact_counts_kernel");
+            if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
+                                                                ws.tile_begin, n_tiles + 1, stream),
+                               "hipcub ExclusiveSum"))
+                return SMX_ERUNTIME;
+        }
         if (n > 0) {
             if (K)
                 hipLaunchKernelGGL(bin_fill_rec_kernel, dim3(nbr), dim3(REC_THREADS), 0, stream, n,

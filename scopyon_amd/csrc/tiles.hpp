@@ -43,6 +43,39 @@ __global__ void compact_counts_kernel(const int* __restrict__ counts, int cs, in
     if (g < n) dense[g] = counts[(size_t)g * cs];
 }
 
+// Exclusive scan of the (strided) tile counters in one workgroup of
+// SCAN_THREADS threads: begin[g] = sum of counts[h * cs] for h < g, g < n.
+// Each thread sums a contiguous run of ceil(n / SCAN_THREADS) counters, the
+// workgroup scans the run totals in LDS, and each thread writes its run's
+// offsets.  Replaces compact + device-wide scan (three launches) up to
+// SCAN_MAX counters.
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_MAX = 65536;
+__global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __restrict__ counts,
+                                                                  int cs, int n,
+                                                                  int* __restrict__ begin) {
+    __shared__ int s_run[SCAN_THREADS];
+    const int t = threadIdx.x;
+    const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int a = t * per, b = min(n, a + per);
+    int sum = 0;
+    for (int g = a; g < b; ++g) sum += counts[(size_t)g * cs];
+    s_run[t] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the run totals
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        const int v = t >= off ? s_run[t - off] : 0;
+        __syncthreads();
+        s_run[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? s_run[t - 1] : 0;
+    for (int g = a; g < b; ++g) {
+        begin[g] = run;
+        run += counts[(size_t)g * cs];
+    }
+}
+
 // One pass before binning: tile_count[g * cs] = 0 for g <= n_tiles and
 // tile_fov[g] = the frame owning global tile g (-1 where none does).  The frames' tile ranges
 // are staged in LDS (up to TILES_INIT_FOVS of them) and searched by bisection

@@ -86,7 +86,9 @@ class GaussianModel:
 
     def _workspace(self, n):
         cap = max(1, 4 * n)
-        need = _lib.lib().smx_gauss_workspace_bytes(n, self.n_tiles, cap)
+        lib = _lib.lib()
+        need = max(lib.smx_gauss_workspace_bytes(n, self.n_tiles, cap),
+                   lib.smx_gauss_backward_workspace_bytes(n))
         if self._ws is None or self._ws.numel() < need or self._cap < cap:
             self._ws = torch.empty(int(need), dtype=torch.uint8, device=self.device)
             self._cap = cap
@@ -139,6 +141,32 @@ class GaussianModel:
                       ctypes.byref(p), self.METHODS[method], cap, mu.data_ptr(), ws.data_ptr(),
                       ws.numel(), self.status.data_ptr(), _device.stream())
         return mu
+
+    def forward_nll(self, x, y, z, I, k, frame_id=None, grad_mu=None, sigma=None, mu=None,
+                    method=None):
+        """Forward image and Poisson NLL in one pass on the matrix cores
+        (method 'mfma-f32' / 'mfma-f16' / 'mfma-bf16'): per-frame float64 loss
+        sum(mu - k log mu) (the same per-pixel terms as nll(), summed per tile
+        in a fixed order) and, into grad_mu, dL/dmu = 1 - k / mu, computed from
+        the tile's accumulators; the image itself is written only into `mu`
+        when given."""
+        method = method or self.method
+        if method not in self.METHODS or method == 'lds':
+            raise ValueError('forward_nll runs the matrix-core forms (mfma-*)')
+        for t, nm in ((x, 'x'), (y, 'y'), (I, 'I'), (k, 'k')):
+            self._check(t, nm)
+        if z is not None:
+            self._check(z, 'z')
+        n = x.numel()
+        fid = self._frame_ids(frame_id, n)
+        ws, cap = self._workspace(n)
+        p = self.params(sigma)
+        _lib.call('smx_sep_gemm_nll', n, x.data_ptr(), y.data_ptr(), _device.ptr(z), I.data_ptr(),
+                  _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes, self.n_tiles,
+                  ctypes.byref(p), self.METHODS[method], cap, _device.ptr(mu), k.data_ptr(),
+                  _device.ptr(grad_mu), self.loss.data_ptr(), ws.data_ptr(), ws.numel(),
+                  self.status.data_ptr(), _device.stream())
+        return self.loss
 
     def nll(self, mu, k, grad_mu=None):
         """Per-frame float64 loss (device tensor) and, into grad_mu, 1 - k/mu."""

@@ -58,6 +58,31 @@ def test_forward_and_nll_match_oracle(gpu, method):
     numpy.testing.assert_allclose(grad.cpu().numpy(), 1.0 - kd.cpu().numpy() / mu_h, rtol=1e-6)
 
 
+@pytest.mark.parametrize('method', ['mfma-f32', 'mfma-bf16'])
+def test_forward_nll_fused_matches_separate(gpu, method):
+    """Forward + NLL in one matrix-core pass: the gradient 1 - k/mu is
+    bit-identical to forward() then nll(), the per-frame loss equal to
+    float64 rounding (the terms are summed per tile, then over tiles), with
+    and without writing the image."""
+    shapes = [(64, 64), (40, 72), (100, 33)]
+    x, y, z, I, fid, k = _case(0, shapes, 60)
+    m = _model(shapes, method)
+    args = [_dev(x), _dev(y), _dev(z), _dev(I)]
+    kd = torch.cat([_dev(a.ravel()) for a in k])
+    mu = m.forward(*args, _dev(fid, torch.int32))
+    grad = torch.empty_like(mu)
+    want = m.nll(mu, kd, grad).clone()
+    for write_mu in (True, False):
+        mu2 = torch.full_like(mu, -1.0) if write_mu else None
+        grad2 = torch.full_like(mu, -7.0)
+        loss = m.forward_nll(*args, kd, _dev(fid, torch.int32), grad_mu=grad2, mu=mu2)
+        assert m.check_status() & ~1 == 0
+        torch.testing.assert_close(loss, want, rtol=1e-12, atol=0)
+        assert torch.equal(grad2, grad)
+        if write_mu:
+            assert torch.equal(mu2, mu)
+
+
 @pytest.mark.parametrize('method', ['lds', 'mfma-f32'])
 def test_many_frames_span_the_binning_window(gpu, method):
     """72 frames of 256^2 (64 tiles each, 4608 in all) with 16 emitters each:
