@@ -375,6 +375,23 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
                        const smx_gauss_params_t* params /* host */, const float* grad_mu,
                        float* gx, float* gy, float* gz, float* gI, double* gsigma,
                        void* workspace, size_t workspace_bytes, hipStream_t stream);
+/* The same gradients from the binning of the forward call that produced
+ * mu (smx_gauss_forward, smx_sep_gemm or smx_sep_gemm_nll with the same n,
+ * emitters, frames, n_tiles, params and capacity, whose workspace is passed
+ * here untouched, on the same stream): each tile stages its window of
+ * grad_mu in LDS and reduces the emitters whose window centre (clamped into
+ * the frame) it holds; dL/dsigma is summed per tile in list order, then over
+ * the tiles.  gx, gy, gz, gI as smx_gauss_backward (bit-identical values);
+ * gsigma the same sum in another fixed order.  status bit 16: the workspace
+ * holds the binning of another sigma, radius, n or n_tiles (results
+ * invalid). */
+int smx_gauss_backward_binned(int n, const float* x, const float* y, const float* z,
+                              const float* I, const int* frame_id, const smx_fov_t* frames_dev,
+                              int nframes, int n_tiles,
+                              const smx_gauss_params_t* params /* host */, long long capacity,
+                              const float* grad_mu, float* gx, float* gy, float* gz, float* gI,
+                              double* gsigma, void* workspace, size_t workspace_bytes,
+                              int* status_dev, hipStream_t stream);
 
 /* ---- Brownian sampling (sampling2.py:33-68, sampling.py:88-119) --------- */
 /* Whole trajectories from caller-drawn normals xi[t][i][d] (and transition

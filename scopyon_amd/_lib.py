@@ -125,6 +125,9 @@ SIGNATURES.update({
     'smx_gauss_poisson_nll': (c_int, [P, P, P, c_int, c_int, P, P, P, c_size_t, P]),
     'smx_gauss_backward': (c_int, [c_int, P, P, P, P, P, P, c_int, ctypes.POINTER(GaussParamsT),
                                    P, P, P, P, P, P, P, c_size_t, P]),
+    'smx_gauss_backward_binned': (c_int, [c_int, P, P, P, P, P, P, c_int, c_int,
+                                          ctypes.POINTER(GaussParamsT), c_longlong, P, P, P, P,
+                                          P, P, P, c_size_t, P, P]),
 })
 SIGNATURES['smx_splat_bin'] = SIGNATURES['smx_splat_expected']
 SIGNATURES['smx_splat_accumulate'] = SIGNATURES['smx_splat_expected']

@@ -7,23 +7,25 @@
 // pixel; the TIRF amplitude exp(-|depth| / penetration_depth) of
 // _epifm.py:1289.  Units here are pixels (x, y, sigma) and micrometres (z).
 //
-// Forward: emitters are binned to 32 x 32 tiles, each tile's list is sorted by
-// emitter index, one wave per 16 x 16 quadrant adds every emitter's
-// contributions in emitter order (deterministic, no float atomics in global
-// memory).  The per-axis pixel factors G(i; x) are evaluated once per emitter
-// and tile chunk into LDS (2 x (2R + 1) erf differences); a contribution is
-// then one product.
+// Forward: emitters are binned to 32 x 32 tiles and each tile's list is
+// sorted by emitter index (deterministic, no float atomics in global memory).
+// The matrix-core form writes a tile as the contraction of per-emitter row
+// and column factors (sep_gemm_kernel); the LDS form adds each emitter's
+// products into LDS accumulators (gauss_forward_kernel).  Window factors come
+// from a float64-built table over sub-pixel phases (gauss_table_kernel).
 //
-// Backward: one 16-lane group per emitter.  With g = dL/dmu over the
-// emitter's (2R+1)^2 window, every gradient is a separable sum
-//   sum_ij g_ij u_i v_j  (u, v in {G, dG/dx, dG/dsigma}),
-// so each lane owns one window column, walks the rows (one coalesced load
-// per row), and the group reduces five dot products with shuffles.
+// Backward: every gradient is a separable sum over the emitter's window,
+//   sum_ij g_ij u_i v_j  (g = dL/dmu, u, v in {G, dG/dx, dG/dsigma}),
+// reduced by a 16-lane group per emitter: each lane owns one window column,
+// walks the rows and the group combines five dot products with shuffles.
+// The standalone form reads g from the image; the binned form reuses the
+// forward's tile lists and reads each tile's window of g from LDS.
 #include "common.hpp"
 #include "scopyon_hip_types.h"
 #include "tiles.hpp"
 
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
 #include <type_traits>
 
 namespace {
@@ -43,7 +45,6 @@ constexpr int PSORT_CAP = 2048;  // tile list sorted in LDS
 constexpr int WMAX = 15;         // largest window (radius 7)
 constexpr int NSLOT = 8;         // 64-pixel parts in flight per wave
 constexpr float INV_SQRT2 = 0.70710678118654752f;
-constexpr float INV_SQRT2PI = 0.39894228040143268f;
 
 // Pixel-integrated unit Gaussian along one axis: (erf(t1) - erf(t0)) / 2 with
 // t = (edge - x) / (sqrt(2) sigma) for the pixel's two edges t0 < t1.  A pixel
@@ -62,43 +63,77 @@ __device__ inline bool finite_coord(float v) { return v > -1.0e6f && v < 1.0e9f;
 // Window factors by sub-pixel phase.  For an emitter at x = floor(x) + d,
 // window pixel b (of 2R + 1) spans [b - R - d, b - R + 1 - d) relative to x,
 // so its factors depend on (d, b) only: G (the pixel-integrated unit
-// Gaussian), dG/dx and dG/dsigma.  GQ + 1 phases d = q / GQ are evaluated in
-// float64 once per sigma (gauss_table_kernel) and read with linear
+// Gaussian), dG/dx and dG/dsigma.  They are evaluated in float64 once per
+// sigma at GQ + 1 phases d = q / GQ (gauss_table_kernel) and read with linear
 // interpolation in d (error ~1e-8 of the peak factor at GQ = 1024 and sigma
-// >= 0.5 px: the second derivative in d is bounded by 0.24 / sigma^2).
+// >= 0.5 px: the second derivative in d is bounded by 0.24 / sigma^2).  Entry
+// (q, b) of each quantity holds the pair (value at q, value at q + 1 minus
+// it), so one 8-byte load and one FMA give an interpolated factor; the three
+// quantities are stored one after the other (G first: the forward reads only
+// that 128 KB block).
 constexpr int GQ = 1024;
-constexpr int GB = 16;            // factor slots per phase (window <= 15)
-constexpr int TABLE_ENTRIES = (GQ + 1) * GB;
+constexpr int GB = 16;                   // factor slots per phase (window <= 15)
+constexpr int TABLE_ENTRIES = GQ * GB;   // per quantity
+constexpr int TABLE_PAIRS = 3 * TABLE_ENTRIES;
 
-__global__ void gauss_table_kernel(smx_gauss_params_t P, float4* __restrict__ tab) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= TABLE_ENTRIES) return;
-    const int q = idx / GB, b = idx % GB;
-    const int R = P.radius, W = 2 * R + 1;
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (b < W) {
-        const double sigma = (double)P.sigma;
-        const double e0 = (double)(b - R) - (double)q / GQ, e1 = e0 + 1.0;
-        const double s = 0.70710678118654752 / sigma;
-        double g;
-        if (e0 >= 0.0) g = 0.5 * (erfc(e0 * s) - erfc(e1 * s));
-        else if (e1 <= 0.0) g = 0.5 * (erfc(-e1 * s) - erfc(-e0 * s));
-        else g = 0.5 * (erf(e1 * s) - erf(e0 * s));
-        const double c = 0.39894228040143268 / sigma;
-        const double p0 = c * exp(-0.5 * (e0 / sigma) * (e0 / sigma));
-        const double p1 = c * exp(-0.5 * (e1 / sigma) * (e1 / sigma));
-        v = make_float4((float)g, (float)(p0 - p1), (float)(-(e1 * p1 - e0 * p0) / sigma), 0.0f);
-    }
-    tab[idx] = v;
+// G, dG/dx and dG/dsigma of window pixel b at phase d (float64)
+__device__ inline void gauss_factors_f64(double sigma, int R, double d, int b, double out[3]) {
+    const double e0 = (double)(b - R) - d, e1 = e0 + 1.0;
+    const double s = 0.70710678118654752 / sigma;
+    double g;
+    if (e0 >= 0.0) g = 0.5 * (erfc(e0 * s) - erfc(e1 * s));
+    else if (e1 <= 0.0) g = 0.5 * (erfc(-e1 * s) - erfc(-e0 * s));
+    else g = 0.5 * (erf(e1 * s) - erf(e0 * s));
+    const double c = 0.39894228040143268 / sigma;
+    const double p0 = c * exp(-0.5 * (e0 / sigma) * (e0 / sigma));
+    const double p1 = c * exp(-0.5 * (e1 / sigma) * (e1 / sigma));
+    out[0] = g;
+    out[1] = p0 - p1;
+    out[2] = -(e1 * p1 - e0 * p0) / sigma;
 }
 
-// (G, dG/dx, dG/dsigma) of window pixel b at sub-pixel offset d in [0, 1).
-__device__ inline float4 table_factor(const float4* __restrict__ tab, float d, int b) {
+// one table entry (q, b) of all three quantities; idx < TABLE_ENTRIES
+__device__ inline void gauss_table_entry(const smx_gauss_params_t& P, int idx,
+                                         float2* __restrict__ tab) {
+    const int q = idx / GB, b = idx % GB;
+    const int R = P.radius, W = 2 * R + 1;
+    double v0[3] = {0.0, 0.0, 0.0}, v1[3] = {0.0, 0.0, 0.0};
+    if (b < W) {
+        gauss_factors_f64((double)P.sigma, R, (double)q / GQ, b, v0);
+        gauss_factors_f64((double)P.sigma, R, (double)(q + 1) / GQ, b, v1);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a = (float)v0[k];
+        tab[k * TABLE_ENTRIES + idx] = make_float2(a, (float)(v1[k] - (double)a));
+    }
+}
+
+__global__ void gauss_table_kernel(smx_gauss_params_t P, float2* __restrict__ tab) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < TABLE_ENTRIES) gauss_table_entry(P, idx, tab);
+}
+
+__device__ inline int table_slot(float d, float& f) {
     const float t = d * (float)GQ;
     const int q = min((int)t, GQ - 1);
-    const float f = t - (float)q;
-    const float4 a = tab[q * GB + b], c = tab[(q + 1) * GB + b];
-    return make_float4(a.x + f * (c.x - a.x), a.y + f * (c.y - a.y), a.z + f * (c.z - a.z), 0.0f);
+    f = t - (float)q;
+    return q * GB;
+}
+
+// G of window pixel b at sub-pixel offset d in [0, 1)
+__device__ inline float table_g(const float2* __restrict__ tab, float d, int b) {
+    float f;
+    const float2 a = tab[table_slot(d, f) + b];
+    return fmaf(f, a.y, a.x);
+}
+
+// (G, dG/dx, dG/dsigma) of window pixel b at sub-pixel offset d in [0, 1)
+__device__ inline float4 table_factor(const float2* __restrict__ tab, float d, int b) {
+    float f;
+    const int o = table_slot(d, f) + b;
+    const float2 g = tab[o], dx = tab[TABLE_ENTRIES + o], ds = tab[2 * TABLE_ENTRIES + o];
+    return make_float4(fmaf(f, g.y, g.x), fmaf(f, dx.y, dx.x), fmaf(f, ds.y, ds.x), 0.0f);
 }
 
 struct TileRange { int r0, r1, c0, c1; };
@@ -116,6 +151,55 @@ __device__ inline bool window_tiles(const smx_fov_t& F, float x, float y, int R,
 }
 
 __device__ inline int frame_of(const int* __restrict__ frame_id, int i) { return frame_id ? frame_id[i] : 0; }
+
+// The tile that owns an emitter in the binned backward: the tile of its
+// window centre pixel clamped into the frame.  The binning lists the emitter
+// in that tile (the clamped centre lies inside the window's clipped range),
+// so every listed emitter has exactly one owner; -1 when the window misses
+// the frame or the tile lies outside the launch.
+__device__ inline int owner_tile(const smx_fov_t& F, float x, float y, int R, int n_tiles) {
+    TileRange t;
+    if (!window_tiles(F, x, y, R, t)) return -1;
+    const int cu = min(max((int)floorf(x), 0), F.nw - 1);
+    const int cv = min(max((int)floorf(y), 0), F.nh - 1);
+    const int tile = F.tile_off + (cu / TS) * F.tiles_c + cv / TS;
+    return tile >= 0 && tile < n_tiles ? tile : -1;
+}
+
+// What a binning was built for (written by gauss_init_kernel, checked by the
+// binned backward).
+struct GaussHeader {
+    float sigma;
+    int radius;
+    int n;
+    int n_tiles;
+};
+
+// Tile counters and the tile -> frame map (blocks below tile_blocks), the
+// phase table and the header (the blocks after them), in one launch.
+__global__ void __launch_bounds__(256) gauss_init_kernel(int nfov,
+                                                         const smx_fov_t* __restrict__ fovs,
+                                                         int n_tiles, int* __restrict__ tile_count,
+                                                         int cs, int* __restrict__ tile_fov,
+                                                         int tile_blocks, smx_gauss_params_t P,
+                                                         int n, float2* __restrict__ tab,
+                                                         GaussHeader* __restrict__ hdr) {
+    __shared__ smx::TilesInitLds L;
+    if ((int)blockIdx.x < tile_blocks) {
+        smx::tiles_init_block(L, blockIdx.x, nfov, fovs, n_tiles, tile_count, cs, tile_fov);
+        return;
+    }
+    const int idx = ((int)blockIdx.x - tile_blocks) * blockDim.x + threadIdx.x;
+    if (idx == 0) {
+        GaussHeader h;
+        h.sigma = P.sigma;
+        h.radius = P.radius;
+        h.n = n;
+        h.n_tiles = n_tiles;
+        *hdr = h;
+    }
+    if (idx < TABLE_ENTRIES) gauss_table_entry(P, idx, tab);
+}
 
 // Binning aggregated per block of BIN_THREADS emitters in LDS (one device
 // atomic per block and tile; a window of BIN_WIN tiles from the first tile of
@@ -226,7 +310,7 @@ struct FwdArgs {
     int nframes;
     const int* tile_fov;
     const int* tile_begin;
-    const int* list;
+    int* list;           // each tile's sorted list is written back in place
     long long capacity;
     smx_gauss_params_t P;
     float* mu;
@@ -275,6 +359,9 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
     } else if (!sorted && threadIdx.x == 0) {
         atomicOr(A.status, 1);
     }
+    // the sorted list goes back in place for the binned backward
+    if (sorted && count > 1)
+        for (int k = threadIdx.x; k < count; k += blockDim.x) A.list[begin + k] = s_sort[k];
 
     for (int base = 0; base < count; base += FCHUNK) {
         const int nc = min(FCHUNK, count - base);
@@ -361,8 +448,19 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
     }
 }
 
+// dL/dmu of the Poisson NLL, 1 - k / mu, with the hardware reciprocal
+// (v_rcp_f32, 1 ulp); every NLL path writes this value.
+__device__ inline float nll_grad(float mu, float k) { return 1.0f - k * __builtin_amdgcn_rcpf(mu); }
+// The fused tile pass's NLL term mu - k log(mu): the logarithm on the
+// hardware log2 (v_log_f32, ~1e-7 relative), the term and its sums in
+// float64.  gauss_nll_kernel evaluates the same term with a float64 log.
+__device__ inline double nll_term(float mu, float k) {
+    const float lg = 0.69314718055994531f * __builtin_amdgcn_logf(mu);
+    return (double)mu - (double)k * (double)lg;
+}
+
 // Poisson NLL per frame, sum over pixels of mu - k log(mu) (float64), and
-// dL/dmu = 1 - k / mu.  Block b of frame f reduces pixels [b * per, (b+1) * per).
+// dL/dmu (nll_grad).  Block b of frame f reduces pixels [b * per, (b+1) * per).
 __global__ void __launch_bounds__(256) gauss_nll_kernel(const float* __restrict__ mu,
                                                         const float* __restrict__ k,
                                                         const smx_fov_t* __restrict__ frames,
@@ -380,7 +478,7 @@ __global__ void __launch_bounds__(256) gauss_nll_kernel(const float* __restrict_
         const float m = mu[o], kk = k[o];
         const double md = (double)(m > 0.0f ? m : 1.0e-30f);
         sum += md - (double)kk * log(md);
-        if (grad) grad[o] = 1.0f - kk / (m > 0.0f ? m : 1.0e-30f);
+        if (grad) grad[o] = nll_grad(m > 0.0f ? m : 1.0e-30f, kk);
     }
     s_red[threadIdx.x] = sum;
     __syncthreads();
@@ -416,8 +514,45 @@ struct BwdArgs {
     float* gz;
     float* gI;
     double* gsig_part;   // per emitter: dL/dsigma contribution (may be NULL)
-    const float4* table; // window factors by phase (gauss_table_kernel)
+    const float2* table; // window factors by phase (gauss_table_kernel)
 };
+
+// Writes gI, gx, gy, gz of emitter e (e >= 0) from the reduced sums pI, px,
+// py and returns Ia * ps (ps: dmu/dsigma . g).
+__device__ inline float emitter_outputs(float pI, float px, float py, float ps, int e, bool valid,
+                                        float ze, float Ie, float inv_depth, float* gx, float* gy,
+                                        float* gz, float* gI) {
+    const float a = valid ? expf(-fabsf(ze) * inv_depth) : 0.0f;
+    const float Ia = Ie * a;
+    if (e >= 0) {
+        const float sgn = ze > 0.0f ? 1.0f : (ze < 0.0f ? -1.0f : 0.0f);
+        if (gI) gI[e] = a * pI;
+        if (gx) gx[e] = Ia * px;
+        if (gy) gy[e] = Ia * py;
+        if (gz) gz[e] = -sgn * inv_depth * Ia * pI;
+    }
+    return Ia * ps;
+}
+
+// Gradients of one emitter from the five separable sums (a 16-lane group,
+// lane j holding window column j; lane 0 writes): column j's terms t1 gy,
+// t2 gy, t1 dgy and t3 gy + t1 sgy are summed by the xor-shuffle tree over
+// strides 8, 4, 2, 1.  Returns Ia * dmu/dsigma . g (lane 0).
+__device__ inline float group_outputs(float t1, float t2, float t3, float gyc, float dgyc,
+                                      float sgyc, int j, int e, bool valid, float ze, float Ie,
+                                      float inv_depth, float* gx, float* gy, float* gz,
+                                      float* gI) {
+    float pI = t1 * gyc, px = t2 * gyc, py = t1 * dgyc, ps = t3 * gyc + t1 * sgyc;
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) {
+        pI += __shfl_xor(pI, m);
+        px += __shfl_xor(px, m);
+        py += __shfl_xor(py, m);
+        ps += __shfl_xor(ps, m);
+    }
+    return emitter_outputs(pI, px, py, ps, j == 0 ? e : -1, valid, ze, Ie, inv_depth, gx, gy, gz,
+                           gI);
+}
 
 __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
     __shared__ float4 s_rowf[16][WMAX];   // per group, window row k: (G, dG/dx, dG/dsigma)
@@ -442,21 +577,15 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
     const int lu = valid ? (int)floorf(xe) - R : 0, lv = valid ? (int)floorf(ye) - R : 0;
     // this lane's window row factors (row lu + j) and column factors (col lv + j):
     // G, dG/dx and dG/dsigma from the phase table
-    float gxr = 0.0f, dgxr = 0.0f, sgxr = 0.0f, gyc = 0.0f, dgyc = 0.0f, sgyc = 0.0f;
+    float4 fx = make_float4(0.0f, 0.0f, 0.0f, 0.0f), fy = fx;
     if (valid && j < W) {
-        const float4 fx = table_factor(A.table, xe - floorf(xe), j);
-        const float4 fy = table_factor(A.table, ye - floorf(ye), j);
-        gxr = fx.x;
-        dgxr = fx.y;
-        sgxr = fx.z;
-        gyc = fy.x;
-        dgyc = fy.y;
-        sgyc = fy.z;
+        fx = table_factor(A.table, xe - floorf(xe), j);
+        fy = table_factor(A.table, ye - floorf(ye), j);
     }
     // the group's row factors go through LDS: one 16-byte broadcast read per
     // row instead of three cross-lane shuffles
     float4* rowf = s_rowf[threadIdx.x >> 4];
-    if (j < W) rowf[j] = make_float4(gxr, dgxr, sgxr, 0.0f);
+    if (j < W) rowf[j] = fx;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -469,28 +598,242 @@ __global__ void __launch_bounds__(256) gauss_backward_kernel(BwdArgs A) {
         float g = 0.0f;
         if (col_in && row >= 0 && row < F.nw)
             g = A.grad_mu[(size_t)F.pix_off + (size_t)row * F.nh + col];
-        t1 += g * fk.x;
-        t2 += g * fk.y;
-        t3 += g * fk.z;
+        t1 = fmaf(g, fk.x, t1);
+        t2 = fmaf(g, fk.y, t2);
+        t3 = fmaf(g, fk.z, t3);
     }
-    float pI = t1 * gyc, px = t2 * gyc, py = t1 * dgyc, ps = t3 * gyc + t1 * sgyc;
+    const float ps = group_outputs(t1, t2, t3, fy.x, fy.y, fy.z, j, e < A.n ? e : -1, valid, ze,
+                                   Ie, A.P.inv_depth, A.gx, A.gy, A.gz, A.gI);
+    if (j == 0 && e < A.n && A.gsig_part) A.gsig_part[e] = (double)ps;
+}
+
+// Binned backward: one workgroup per tile of the forward's binning.  The
+// tile's window of dL/dmu (the tile plus R pixels on every side, zero off
+// the frame) is staged in LDS once; the tile's list (as the forward left it:
+// in emitter order when it was sorted) is filtered to the emitters this tile
+// owns (owner_tile), and each owned emitter is reduced by a 16-lane group
+// exactly as gauss_backward_kernel does, reading LDS instead of the image.
+// Per tile the dL/dsigma terms are summed in list order with a fixed tree.
+// Workgroups past n_tiles write zero gradients for the emitters no tile owns.
+constexpr int HALO_MAX = TS + 2 * 7;
+constexpr int BT_THREADS = 256;
+constexpr int BT_GROUPS = BT_THREADS / 4;   // emitters per round (four lanes each)
+
+struct BwdTileArgs {
+    int n;
+    const float* x;
+    const float* y;
+    const float* z;
+    const float* I;
+    const int* frame_id;
+    const smx_fov_t* frames;
+    int nframes;
+    int n_tiles;
+    const int* tile_fov;
+    const int* tile_begin;
+    const int* list;
+    long long capacity;
+    smx_gauss_params_t P;
+    const float2* table;
+    const GaussHeader* hdr;
+    const float* grad_mu;
+    float* gx;
+    float* gy;
+    float* gz;
+    float* gI;
+    double* tile_gsig;
+    int* status;
+};
+
+__global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTileArgs A) {
+    // the tile's window of dL/dmu with R zero guard rows above and below
+    // (rows of an off-frame window centre past the window are off the frame)
+    __shared__ float s_g[(HALO_MAX + 14) * HALO_MAX];
+    __shared__ float4 s_em[BT_THREADS];       // owned emitters: x, y, z, I
+    __shared__ int s_own[BT_THREADS];
+    __shared__ double s_gs[BT_THREADS];
+    // per emitter of a round, window row k: G, dG/dx, dG/dsigma
+    __shared__ float s_rowf[BT_GROUPS][GB][3];
+    __shared__ int s_wcnt[BT_THREADS / 64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int R = A.P.radius, W = 2 * R + 1;
+    if ((int)blockIdx.x >= A.n_tiles) {
+        const int sweep = gridDim.x - A.n_tiles;
+        for (long long e = (long long)(blockIdx.x - A.n_tiles) * BT_THREADS + tid; e < A.n;
+             e += (long long)sweep * BT_THREADS) {
+            const int f = frame_of(A.frame_id, (int)e);
+            const bool owned = f >= 0 && f < A.nframes &&
+                               owner_tile(A.frames[f], A.x[e], A.y[e], R, A.n_tiles) >= 0;
+            if (!owned) {
+                if (A.gI) A.gI[e] = 0.0f;
+                if (A.gx) A.gx[e] = 0.0f;
+                if (A.gy) A.gy[e] = 0.0f;
+                if (A.gz) A.gz[e] = 0.0f;
+            }
+        }
+        return;
+    }
+    const int tile = blockIdx.x;
+    const int fr = A.tile_fov[tile];
+    const int begin = A.tile_begin[tile];
+    int count = A.tile_begin[tile + 1] - begin;
+    if (tile == 0 && tid == 0) {
+        const GaussHeader h = *A.hdr;
+        if (h.sigma != A.P.sigma || h.radius != R || h.n != A.n || h.n_tiles != A.n_tiles)
+            atomicOr(A.status, 16);
+    }
+    if (fr < 0 || fr >= A.nframes) {
+        if (tid == 0) A.tile_gsig[tile] = 0.0;
+        return;
+    }
+    const smx_fov_t F = A.frames[fr];
+    const int local = tile - F.tile_off;
+    const int tr = local / F.tiles_c, tc = local % F.tiles_c;
+    const int HS = TS + 2 * R;
+    const int hr0 = tr * TS - R, hc0 = tc * TS - R;
+    // staged rows r < HS + 2 R: frame row hr0 - R + r (guard rows zero),
+    // one row per wave and iteration, one column per lane
+    for (int r = wave; r < HS + 2 * R; r += BT_THREADS / 64) {
+        const int pr = hr0 - R + r, pc = hc0 + lane;
+        if (lane < HS) {
+            const bool ok = r >= R && r < HS + R && pr >= 0 && pr < F.nw && pc >= 0 && pc < F.nh;
+            s_g[r * HS + lane] = ok ? A.grad_mu[(size_t)F.pix_off + (size_t)pr * F.nh + pc] : 0.0f;
+        }
+    }
+    if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
+    const int* list = A.list + begin;
+    // four lanes per emitter: lane q of group grp holds window slots
+    // 4 q .. 4 q + 3 of both axes: the factors of its rows (into LDS) and of
+    // its columns (registers) from two 16-byte table loads per axis and
+    // quantity (one cache line per emitter, axis and quantity), then the
+    // sums over all rows for its four columns
+    const int grp = tid >> 2, q = tid & 3;
+    const float4* tab4 = reinterpret_cast<const float4*>(A.table);
+    double gsum = 0.0;
+    for (int sb = 0; sb < count; sb += BT_THREADS) {
+        const int ns = min(BT_THREADS, count - sb);
+        int e = -1;
+        bool own = false;
+        float4 em = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (tid < ns) {
+            e = list[sb + tid];
+            em = make_float4(A.x[e], A.y[e], A.z[e], A.I[e]);
+            own = owner_tile(F, em.x, em.y, R, A.n_tiles) == tile;
+        }
+        // order-preserving compaction of the owned entries
+        const unsigned long long mask = __ballot(own);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wcnt[wave] = __popcll(mask);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wave; ++w) off += s_wcnt[w];
+        int nown = 0;
 #pragma unroll
-    for (int m = 8; m > 0; m >>= 1) {
-        pI += __shfl_xor(pI, m);
-        px += __shfl_xor(px, m);
-        py += __shfl_xor(py, m);
-        ps += __shfl_xor(ps, m);
+        for (int w = 0; w < BT_THREADS / 64; ++w) nown += s_wcnt[w];
+        if (own) {
+            s_own[off + rank] = e;
+            s_em[off + rank] = em;
+        }
+        __syncthreads();
+        for (int ob = 0; ob < nown; ob += BT_GROUPS) {
+            if (ob + wave * (BT_GROUPS / 4) >= nown) continue;   // this wave's 16 emitters: none
+            const int o = ob + grp;
+            const bool valid = o < nown;
+            const float4 v = s_em[min(o, nown - 1)];
+            const int ee = valid ? s_own[o] : -1;
+            const float fxl = floorf(v.x), fyl = floorf(v.y);
+            const int lu = (int)fxl - R, lv = (int)fyl - R;
+            float fu, fv;
+            const int su = table_slot(v.x - fxl, fu) + 4 * q, sv = table_slot(v.y - fyl, fv) + 4 * q;
+            float gy[4], dgy[4], sgy[4];
+            int hcol[4];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float4* tu = tab4 + ((c * TABLE_ENTRIES + su) >> 1);
+                const float4* tv = tab4 + ((c * TABLE_ENTRIES + sv) >> 1);
+                const float4 u0 = tu[0], u1 = tu[1], v0 = tv[0], v1 = tv[1];
+                const float ru[4] = {fmaf(fu, u0.y, u0.x), fmaf(fu, u0.w, u0.z),
+                                     fmaf(fu, u1.y, u1.x), fmaf(fu, u1.w, u1.z)};
+                const float rv[4] = {fmaf(fv, v0.y, v0.x), fmaf(fv, v0.w, v0.z),
+                                     fmaf(fv, v1.y, v1.x), fmaf(fv, v1.w, v1.z)};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (4 * q + i < W) s_rowf[grp][4 * q + i][c] = ru[i];
+                    (c == 0 ? gy : c == 1 ? dgy : sgy)[i] = rv[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // a column outside the staged block (off the frame) or past
+                // the window reads a clamped cell and has zero factors
+                const int jj = 4 * q + i, hc = lv + jj - hc0;
+                const bool col_in = valid && jj < W && hc >= 0 && hc < HS;
+                gy[i] = col_in ? gy[i] : 0.0f;
+                dgy[i] = col_in ? dgy[i] : 0.0f;
+                sgy[i] = col_in ? sgy[i] : 0.0f;
+                hcol[i] = min(max(hc, 0), HS - 1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // staged row of window row 0: lu - (hr0 - R) in [0, HS + R] for
+            // every owned emitter
+            const float* grow = s_g + (valid ? (lu - hr0 + R) : 0) * HS;
+            float t1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, t2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            float t3[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < WMAX; ++k) {
+                if (k < W) {
+                    const float f0 = s_rowf[grp][k][0], f1 = s_rowf[grp][k][1];
+                    const float f2 = s_rowf[grp][k][2];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const float g = grow[k * HS + hcol[i]];
+                        t1[i] = fmaf(g, f0, t1[i]);
+                        t2[i] = fmaf(g, f1, t2[i]);
+                        t3[i] = fmaf(g, f2, t3[i]);
+                    }
+                }
+            }
+            float vI[4], vx[4], vy[4], vs[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                vI[i] = t1[i] * gy[i];
+                vx[i] = t2[i] * gy[i];
+                vy[i] = t1[i] * dgy[i];
+                vs[i] = t3[i] * gy[i] + t1[i] * sgy[i];
+            }
+            float pI = (vI[0] + vI[1]) + (vI[2] + vI[3]);
+            float px = (vx[0] + vx[1]) + (vx[2] + vx[3]);
+            float py = (vy[0] + vy[1]) + (vy[2] + vy[3]);
+            float ps = (vs[0] + vs[1]) + (vs[2] + vs[3]);
+#pragma unroll
+            for (int m = 1; m < 4; m <<= 1) {
+                pI += __shfl_xor(pI, m);
+                px += __shfl_xor(px, m);
+                py += __shfl_xor(py, m);
+                ps += __shfl_xor(ps, m);
+            }
+            const float gs = emitter_outputs(pI, px, py, ps, q == 0 ? ee : -1, valid, v.z, v.w,
+                                             A.P.inv_depth, A.gx, A.gy, A.gz, A.gI);
+            if (valid && q == 0) s_gs[o] = (double)gs;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < BT_THREADS / 64; ++r)
+                if (r * 64 + lane < nown) v += s_gs[r * 64 + lane];
+#pragma unroll
+            for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+            gsum += v;
+        }
+        __syncthreads();
     }
-    if (j == 0 && e < A.n) {
-        const float a = valid ? expf(-fabsf(ze) * A.P.inv_depth) : 0.0f;
-        const float Ia = Ie * a;
-        const float sgn = ze > 0.0f ? 1.0f : (ze < 0.0f ? -1.0f : 0.0f);
-        if (A.gI) A.gI[e] = a * pI;
-        if (A.gx) A.gx[e] = Ia * px;
-        if (A.gy) A.gy[e] = Ia * py;
-        if (A.gz) A.gz[e] = -sgn * A.P.inv_depth * Ia * pI;
-        if (A.gsig_part) A.gsig_part[e] = (double)(Ia * ps);
-    }
+    if (tid == 0) A.tile_gsig[tile] = gsum;
 }
 
 // Deterministic float64 sum (fixed assignment of values to threads and a
@@ -514,56 +857,45 @@ __global__ void __launch_bounds__(256) sum_f64_kernel(const double* __restrict__
 // --------------------------------------------------------------------------
 // Separable splat on the matrix cores.  Per 32 x 32 tile the image is the
 // contraction  mu_tile = A B,  A[i][e] = amp_e G(i; x_e),  B[e][j] = G(j; y_e)
-// over the tile's emitters e (zero outside each emitter's window): one
-// 32 x 32 MFMA accumulator per tile, emitters streamed 16 at a time in index
-// order.  Inputs are rounded to the operand type (float32, float16 or
-// bfloat16); accumulation is float32.
+// over the tile's emitters e (zero outside each emitter's window): the tile's
+// sorted list is staged SG_STAGE emitters at a time (window origin, phases,
+// amplitude in LDS), each of the SG_WAVES waves takes every SG_WAVES-th chunk
+// of 16 emitters, builds its factors from the phase table and adds them into
+// its own 32 x 32 MFMA accumulator, and the waves' accumulators are summed in
+// a fixed order at the end.  Inputs are rounded to the operand type (float32,
+// float16 or bfloat16); accumulation is float32.
 // --------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-// Per emitter: the window origin and the (2R+1)-pixel factors along each
-// axis from the phase table (gauss_table_kernel).  32 lanes per emitter: lane
-// t handles window pixel t % 16 of axis t / 16; a block of 256 threads takes
-// 8 emitters at a time, striding over the launch.
-__global__ void __launch_bounds__(256) sep_factor_kernel(int n, const float* __restrict__ x,
-                                                         const float* __restrict__ y,
-                                                         const float* __restrict__ z,
-                                                         const float* __restrict__ I,
-                                                         smx_gauss_params_t P,
-                                                         const float4* __restrict__ tab,
-                                                         float* __restrict__ fac,
-                                                         int2* __restrict__ lo_out) {
-    const int t = threadIdx.x & 31, ax = t >> 4, b = t & 15;
-    const int R = P.radius, W = 2 * R + 1;
-    for (int e = blockIdx.x * 8 + (threadIdx.x >> 5); e < n; e += gridDim.x * 8) {
-        const float pos = ax ? y[e] : x[e];
-        const bool fin = finite_coord(pos);
-        const float fl = floorf(pos);
-        float g = 0.0f;
-        if (fin && b < W) g = table_factor(tab, pos - fl, b).x;
-        float amp = 1.0f;
-        if (ax == 0) amp = I[e] * (z ? expf(-fabsf(z[e]) * P.inv_depth) : 1.0f);
-        fac[(size_t)e * 32 + t] = ax == 0 ? amp * g : g;
-        if (t == 0) {
-            const float xv = x[e], yv = y[e];
-            lo_out[e] = make_int2(finite_coord(xv) ? (int)floorf(xv) - R : 0,
-                                  finite_coord(yv) ? (int)floorf(yv) - R : 0);
-        }
-    }
-}
+constexpr int SG_WAVES = 4;
+constexpr int SG_THREADS = 64 * SG_WAVES;
+constexpr int SG_STAGE = SG_THREADS;
+constexpr int SG_SORT_CAP = PSORT_CAP;   // tile lists sorted in LDS (longer: status bit 1)
+constexpr int SG_RANK_MAX = 64;          // lists rank-sorted by one wave (longer: bitonic)
+// Factor rows of a chunk emitter: [0] = 0, [1 .. W] = amp * G along the rows,
+// [W + 1 ..] = 0, then the same 18 slots along the columns; an MFMA operand
+// is row[clamp(pixel - window origin, -1, W) + 1], zero outside the window
+// with no mask.
+constexpr int SG_AXIS = 18;
+constexpr int SG_ROW = 2 * SG_AXIS;
 
 struct SepArgs {
-    const float* fac;
-    const int2* lo;
+    const float* x;
+    const float* y;
+    const float* z;      // may be NULL (amplitude I)
+    const float* I;
+    const float2* tab;   // G block of the phase table
+    float inv_depth;
+    int R;
+    int W;
     const smx_fov_t* frames;
     int nframes;
     const int* tile_fov;
     const int* tile_begin;
-    const int* list;
+    int* list;           // each tile's sorted list is written back in place
     long long capacity;
-    int W;
     float background;
     float* mu;           // may be NULL when the NLL is fused (NLL)
     int* status;
@@ -572,172 +904,266 @@ struct SepArgs {
     double* tile_nll;    // NLL: sum over the tile of mu - k log(mu)
 };
 
+struct SepLds {
+    int sort[SG_SORT_CAP];                // the tile's list in index order
+    union alignas(16) {
+        int raw[SG_RANK_MAX];             // the list as binned (rank sort input)
+        float red[16 * 64];               // the waves' accumulators, summed in turn
+    } u;
+    int2 lo[SG_STAGE];                    // window origin (frame pixels)
+    float2 ph[SG_STAGE];                  // sub-pixel phases
+    float amp[SG_STAGE];
+    float fac[SG_WAVES][16][SG_ROW];       // per wave: one chunk's factors (padded rows)
+    double nll[SG_WAVES];
+};
+
+// Rank sort of a tile list of 2 <= count <= SG_SORT_CAP distinct entries by
+// the whole workgroup: every entry's rank is the number of smaller entries
+// (broadcast LDS reads, no dependent shuffle stages).  raw[] must hold count
+// rounded up to 4; out[rank] = entry.
+__device__ inline void rank_sort(const int* __restrict__ list, int count, int* raw, int* out) {
+    const int c4 = (count + 3) & ~3;
+    for (int k = threadIdx.x; k < c4; k += blockDim.x) raw[k] = k < count ? list[k] : 0x7fffffff;
+    __syncthreads();
+    const int4* raw4 = reinterpret_cast<const int4*>(raw);
+    for (int p = threadIdx.x; p < count; p += blockDim.x) {
+        const int v = raw[p];
+        int r = 0;
+        for (int q = 0; q < (c4 >> 2); ++q) {
+            const int4 w = raw4[q];
+            r += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
+        }
+        out[r] = v;
+    }
+    __syncthreads();
+}
+
 template <typename T> struct Frag8 { typedef T type __attribute__((ext_vector_type(8))); };
 
 // DT: operand type, 0: float32 (32x32x2), 1: float16 (32x32x16), 2: bfloat16
-// (32x32x16).  NLL: the Poisson NLL of the tile (float64 log, the formula of
-// gauss_nll_kernel) and its image gradient in the epilogue, from the
-// accumulators, instead of a pass over the written image.
+// (32x32x16).  NLL: the Poisson NLL of the tile (nll_term, summed in float64)
+// and its image gradient (nll_grad) in the epilogue, from the accumulators,
+// instead of a pass over the written image.
 template <int DT, bool NLL>
-__global__ void __launch_bounds__(64) sep_gemm_kernel(SepArgs A) {
-    __shared__ int s_sort[PSORT_CAP];
-    __shared__ float s_fac[16][33];
-    __shared__ int2 s_lo[16];
+__global__ void __launch_bounds__(SG_THREADS) __attribute__((amdgpu_waves_per_eu(6, 8)))
+sep_gemm_kernel(SepArgs A) {
+    __shared__ SepLds L;
     const int tile = blockIdx.x;
+    // independent loads first: frame, list bounds
     const int fr = A.tile_fov[tile];
+    const int begin = A.tile_begin[tile];
+    int count = A.tile_begin[tile + 1] - begin;
     if (fr < 0 || fr >= A.nframes) return;
     const smx_fov_t F = A.frames[fr];
     const int local = tile - F.tile_off;
     const int tr = local / F.tiles_c, tc = local % F.tiles_c;
-    const int l = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int m = l & 31, half = l >> 5;
     const int row = tr * TS + m, col = tc * TS + m;   // A row / B column of this lane
+    const int R = A.R, W = A.W;
+    // this wave's outputs: rows 8 wave + 4 half + i (i < 4), column m
+    const int pj = tc * TS + m;
+    float kk[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (NLL && pj < F.nh) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int pi = tr * TS + 8 * wave + 4 * half + i;
+            if (pi < F.nw) kk[i] = A.k[(size_t)F.pix_off + (size_t)pi * F.nh + pj];
+        }
+    }
 
-    const int begin = A.tile_begin[tile];
-    int count = A.tile_begin[tile + 1] - begin;
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
-    const int* list = A.list + begin;
-    const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && count <= WAVE_SORT_MAX) {
-        wave_sort_list(list, count, l, s_sort);
+    int* list = A.list + begin;
+    const bool sorted = count <= SG_SORT_CAP;
+    if (count >= 2 && sorted) {
+        // short lists: rank sort by every thread; up to WAVE_SORT_MAX: one
+        // wave's register bitonic sort; longer: bitonic sort in LDS
+        if (count <= SG_RANK_MAX) {
+            rank_sort(list, count, L.u.raw, L.sort);
+        } else if (count <= WAVE_SORT_MAX) {
+            if (wave == 0) wave_sort_list(list, count, l, L.sort);
+            __syncthreads();
+        } else {
+            int np2 = 1;
+            while (np2 < count) np2 <<= 1;
+            for (int k = tid; k < np2; k += SG_THREADS) L.sort[k] = k < count ? list[k] : 0x7fffffff;
+            __syncthreads();
+            lds_bitonic_sort(L.sort, np2);
+        }
+        // the sorted list goes back in place for the binned backward
+        for (int k = tid; k < count; k += SG_THREADS) list[k] = L.sort[k];
+    } else if (count == 1) {
+        if (tid == 0) L.sort[0] = list[0];
         __syncthreads();
-    } else if (count >= 1 && sorted) {
-        int np2 = 1;
-        while (np2 < count) np2 <<= 1;
-        for (int k = l; k < np2; k += 64) s_sort[k] = k < count ? list[k] : 0x7fffffff;
-        __syncthreads();
-        lds_bitonic_sort(s_sort, np2);
-    } else if (!sorted && l == 0) {
+    } else if (!sorted && tid == 0) {
         atomicOr(A.status, 1);
     }
+
     f32x16 acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    const int W = A.W;
-    // factors of chunk `base` (16 emitters x 32 floats, 8 per lane) are
-    // fetched into registers one chunk ahead, so their latency overlaps the
-    // previous chunk's fragment build and MFMA
-    float pre[8];
-    int2 pre_lo = make_int2(0, 0);   // lanes 0..15: window origin of emitter `l`
-    // branch-free: indices clamped to the list, out-of-chunk values masked,
-    // so the eight loads issue back to back
-    auto fetch = [&](int base) {
-        const int last = count - 1;
-        int e[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int k = 2 * r + (l >> 5);
-            e[r] = sorted ? s_sort[min(base + k, last)] : list[min(base + k, last)];
+    float (*fac)[SG_ROW] = L.fac[wave];
+    // leading zero of both axes of every row (never overwritten)
+    if (l < 32) fac[l >> 1][(l & 1) * SG_AXIS] = 0.0f;
+    // factor build: lane (k, ax, hb) = (l / 4, (l / 2) % 2, l % 2) writes
+    // window pixels 8 hb .. 8 hb + 7 of axis ax of chunk emitter k: four
+    // 16-byte table loads (value, slope pairs; zero pairs past the window)
+    const int fk = l >> 2, fax = (l >> 1) & 1, fhb = l & 1;
+    const float4* tab4 = reinterpret_cast<const float4*>(A.tab);
+    for (int sb = 0; sb < count; sb += SG_STAGE) {
+        const int ns = min(SG_STAGE, count - sb);
+        if (tid < ns) {
+            const int e = sorted ? L.sort[sb + tid] : list[sb + tid];
+            const float xe = A.x[e], ye = A.y[e];
+            const float ie = A.I[e], ze = A.z ? A.z[e] : 0.0f;
+            const float fx = floorf(xe), fy = floorf(ye);
+            const bool fin = finite_coord(xe) && finite_coord(ye);
+            L.lo[tid] = fin ? make_int2((int)fx - R, (int)fy - R) : make_int2(0, 0);
+            L.ph[tid] = fin ? make_float2(xe - fx, ye - fy) : make_float2(0.0f, 0.0f);
+            L.amp[tid] = fin ? ie * (A.z ? expf(-fabsf(ze) * A.inv_depth) : 1.0f) : 0.0f;
         }
-        const int el = sorted ? s_sort[min(base + (l & 15), last)] : list[min(base + (l & 15), last)];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) pre[r] = A.fac[(size_t)e[r] * 32 + (l & 31)];
-        pre_lo = A.lo[el];
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-            if (base + 2 * r + (l >> 5) > last) pre[r] = 0.0f;
-    };
-    if (count > 0) fetch(0);
-    for (int base = 0; base < count; base += 16) {
-        const int nk = min(16, count - base);
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const int idx = r * 64 + l, k = idx >> 5, t = idx & 31;
-            s_fac[k][t] = pre[r];
-        }
-        if (l < nk) s_lo[l] = pre_lo;
         __syncthreads();
-        if (base + 16 < count) fetch(base + 16);
-        // operands: window-origin reads first, then clamped factor reads,
-        // masked outside the window (no branches between the LDS reads)
-        if (DT == 0) {
-            int2 o[8];
+        const int nch = (ns + 15) >> 4;
+        // this lane's table pairs, interpolation weight and scale (amplitude,
+        // 1, or 0 past the chunk) of chunk c, loaded one chunk ahead of use
+        float4 gq[4];
+        float gf = 0.0f, gs = 0.0f;
+        auto load_g = [&](int c) {
+            const int kb = c * 16, nk = min(16, ns - kb);
+            const int k = min(fk, nk - 1);
+            const float2 ph = L.ph[kb + k];
+            float f;
+            const int slot = table_slot(fax ? ph.y : ph.x, f) + fhb * 8;
+            const float4* src = tab4 + (slot >> 1);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) o[j] = s_lo[2 * j + half];
+            for (int i = 0; i < 4; ++i) gq[i] = src[i];
+            gf = f;
+            gs = fk < nk ? (fax ? 1.0f : L.amp[kb + k]) : 0.0f;
+        };
+        if (wave < nch) load_g(wave);
+        for (int c = wave; c < nch; c += SG_WAVES) {
+            const int kb = c * 16;
+            float v[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = 2 * j + half;
-                const int ru = row - o[j].x, cv = col - o[j].y;
-                const bool ok = k < nk;
-                const float a = s_fac[k][min(max(ru, 0), 15)];
-                const float b = s_fac[k][16 + min(max(cv, 0), 15)];
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                    (ok && (unsigned)ru < (unsigned)W) ? a : 0.0f,
-                    (ok && (unsigned)cv < (unsigned)W) ? b : 0.0f, acc, 0, 0, 0);
+            for (int i = 0; i < 4; ++i) {
+                v[2 * i] = fmaf(gf, gq[i].y, gq[i].x) * gs;
+                v[2 * i + 1] = fmaf(gf, gq[i].w, gq[i].z) * gs;
             }
-        } else {
-            typedef typename std::conditional<DT == 1, _Float16, __bf16>::type E;
-            typedef typename Frag8<E>::type V8;
-            V8 a, b;
-            int2 o[8];
+            if (c + SG_WAVES < nch) load_g(c + SG_WAVES);
+            float* dst = &fac[fk][fax * SG_AXIS + 1 + fhb * 8];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) o[t] = s_lo[half * 8 + t];
+            for (int i = 0; i < 8; ++i) dst[i] = v[i];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // operands: row[clamp(offset, -1, W) + 1] (rows past the chunk are zero)
+            if (DT == 0) {
+                int2 o[8];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int k = half * 8 + t;
-                const int ru = row - o[t].x, cv = col - o[t].y;
-                const bool ok = k < nk;
-                const float av = s_fac[k][min(max(ru, 0), 15)];
-                const float bv = s_fac[k][16 + min(max(cv, 0), 15)];
-                a[t] = (E)((ok && (unsigned)ru < (unsigned)W) ? av : 0.0f);
-                b[t] = (E)((ok && (unsigned)cv < (unsigned)W) ? bv : 0.0f);
+                for (int j = 0; j < 8; ++j) o[j] = L.lo[kb + 2 * j + half];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = 2 * j + half;
+                    const int ia = min(max(row - o[j].x, -1), W) + 1;
+                    const int ib = min(max(col - o[j].y, -1), W) + 1;
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(fac[k][ia], fac[k][SG_AXIS + ib],
+                                                               acc, 0, 0, 0);
+                }
+            } else {
+                typedef typename std::conditional<DT == 1, _Float16, __bf16>::type E;
+                typedef typename Frag8<E>::type V8;
+                V8 av, bv;
+                int2 o[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) o[t] = L.lo[kb + half * 8 + t];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int k = half * 8 + t;
+                    const int ia = min(max(row - o[t].x, -1), W) + 1;
+                    const int ib = min(max(col - o[t].y, -1), W) + 1;
+                    av[t] = (E)fac[k][ia];
+                    bv[t] = (E)fac[k][SG_AXIS + ib];
+                }
+                if constexpr (DT == 1) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
+                else acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
             }
-            if constexpr (DT == 1) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc, 0, 0, 0);
-            else acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         __syncthreads();
     }
-    // acc[i] holds row 8 (i / 4) + 4 half + i % 4, column m of the tile
-    const int pj = tc * TS + m;
+    // acc[i] holds row 8 (i / 4) + 4 half + i % 4, column m of the tile.  The
+    // waves' accumulators are added into red[] in wave order, then wave w
+    // finishes rows 8 w .. 8 w + 7.
+#pragma unroll
+    for (int w = 0; w < SG_WAVES; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                L.u.red[i * 64 + l] = w == 0 ? acc[i] : L.u.red[i * 64 + l] + acc[i];
+        }
+        __syncthreads();
+    }
     double part = 0.0;
     if (pj < F.nh) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int pi = tr * TS + 8 * (i >> 2) + 4 * half + (i & 3);
+        for (int i = 0; i < 4; ++i) {
+            const int pi = tr * TS + 8 * wave + 4 * half + i;
             if (pi >= F.nw) continue;
+            const float v = L.u.red[(4 * wave + i) * 64 + l];
             const size_t o = (size_t)F.pix_off + (size_t)pi * F.nh + pj;
-            const float mv = A.background + acc[i];
+            const float mv = A.background + v;
             if (A.mu) A.mu[o] = mv;
             if (NLL) {
-                const float kk = A.k[o];
                 const float mc = mv > 0.0f ? mv : 1.0e-30f;
-                const double md = (double)mc;
-                part += md - (double)kk * log(md);
-                if (A.grad_mu) A.grad_mu[o] = 1.0f - kk / mc;
+                part += nll_term(mc, kk[i]);
+                if (A.grad_mu) A.grad_mu[o] = nll_grad(mc, kk[i]);
             }
         }
     }
     if (NLL) {
-        // fixed reduction tree over the wave: the tile sum is deterministic
+        // fixed reduction trees (over the wave, then the waves in order): the
+        // tile sum is deterministic
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-        if (l == 0) A.tile_nll[tile] = part;
+        if (l == 0) L.nll[wave] = part;
+        __syncthreads();
+        if (tid == 0) {
+            double t = L.nll[0];
+#pragma unroll
+            for (int w = 1; w < SG_WAVES; ++w) t += L.nll[w];
+            A.tile_nll[tile] = t;
+        }
     }
 }
 
-// loss[f] = sum of frame f's tile sums in tile order.
-__global__ void gauss_tile_nll_finish_kernel(const smx_fov_t* __restrict__ frames, int nframes,
-                                             const double* __restrict__ tile_nll,
-                                             double* __restrict__ loss) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+// loss[f] = sum of frame f's tile sums (one wave per frame: lanes take the
+// tiles in a fixed stride, then a fixed shuffle tree).
+__global__ void __launch_bounds__(256) gauss_tile_nll_finish_kernel(
+        const smx_fov_t* __restrict__ frames, int nframes, const double* __restrict__ tile_nll,
+        double* __restrict__ loss) {
+    const int f = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (f >= nframes) return;
     const smx_fov_t F = frames[f];
+    const int nt = F.tiles_r * F.tiles_c;
     double s = 0.0;
-    for (int t = 0; t < F.tiles_r * F.tiles_c; ++t) s += tile_nll[F.tile_off + t];
-    loss[f] = s;
+    for (int t = lane; t < nt; t += 64) s += tile_nll[F.tile_off + t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) loss[f] = s;
 }
 
 struct GaussWorkspace {
-    int* tile_count;   // (n_tiles + 1) x COUNT_STRIDE (padded counters)
-    int* tile_dense;   // n_tiles + 1
-    int* tile_begin;   // n_tiles + 1
-    int* tile_fov;     // n_tiles
-    int* list;         // capacity
-    double* gsig;      // n
-    float* fac;        // n x 32: amp * G along rows (16), G along columns (16)
-    int2* lo;          // n: window origin
-    float4* table;     // TABLE_ENTRIES window factors by phase
-    double* tile_nll;  // n_tiles: fused NLL per tile
+    int* tile_count;     // (n_tiles + 1) x COUNT_STRIDE (padded counters)
+    int* tile_dense;     // n_tiles + 1
+    int* tile_begin;     // n_tiles + 1
+    int* tile_fov;       // n_tiles
+    int* list;           // capacity
+    float2* table;       // TABLE_PAIRS window factors by phase
+    GaussHeader* hdr;    // what the binning and table were built for
+    double* tile_nll;    // n_tiles: fused NLL per tile
+    double* tile_gsig;   // n_tiles: binned backward's dL/dsigma per tile
     void* cub_tmp;
     size_t cub_bytes;
 };
@@ -750,25 +1176,24 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
-    size_t o_sig = take(sizeof(double) * (size_t)(n > 0 ? n : 1));
-    size_t o_fac = take(sizeof(float) * 32 * (size_t)(n > 0 ? n : 1));
-    size_t o_lo = take(sizeof(int2) * (size_t)(n > 0 ? n : 1));
-    size_t o_tab = take(sizeof(float4) * (size_t)TABLE_ENTRIES);
+    size_t o_tab = take(sizeof(float2) * (size_t)TABLE_PAIRS);
+    size_t o_hdr = take(sizeof(GaussHeader));
     size_t o_tnll = take(sizeof(double) * (size_t)n_tiles);
+    size_t o_tgs = take(sizeof(double) * (size_t)n_tiles);
     size_t cb = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cb, (int*)nullptr, (int*)nullptr, n_tiles + 1);
     size_t o_cub = take(cb);
+    (void)n;
     if (ws && base) {
         ws->tile_count = (int*)(base + o_count);
         ws->tile_dense = (int*)(base + o_dense);
         ws->tile_begin = (int*)(base + o_begin);
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
-        ws->gsig = (double*)(base + o_sig);
-        ws->fac = (float*)(base + o_fac);
-        ws->lo = (int2*)(base + o_lo);
-        ws->table = (float4*)(base + o_tab);
+        ws->table = (float2*)(base + o_tab);
+        ws->hdr = (GaussHeader*)(base + o_hdr);
         ws->tile_nll = (double*)(base + o_tnll);
+        ws->tile_gsig = (double*)(base + o_tgs);
         ws->cub_tmp = base + o_cub;
         ws->cub_bytes = cb;
     }
@@ -799,25 +1224,31 @@ size_t smx_gauss_workspace_bytes(int n, int n_tiles, long long capacity) {
 
 size_t smx_gauss_backward_workspace_bytes(int n) {
     if (n < 0) return 0;
-    return backward_table_offset(n) + sizeof(float4) * (size_t)TABLE_ENTRIES;
+    return backward_table_offset(n) + sizeof(float2) * (size_t)TABLE_PAIRS;
 }
 
 }  // extern "C"
 
 namespace {
 
-// Tile binning shared by both forward forms: count -> scan -> fill.
+// Tile binning shared by every forward form: init (counters, tile -> frame
+// map, phase table) -> count -> scan -> fill.
 int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
-              const smx_fov_t* frames_dev, int nframes, int n_tiles, int R, long long capacity,
-              GaussWorkspace& ws, int* status_dev, hipStream_t stream) {
+              const smx_fov_t* frames_dev, int nframes, int n_tiles,
+              const smx_gauss_params_t& P, long long capacity, GaussWorkspace& ws,
+              int* status_dev, hipStream_t stream) {
+    const int R = P.radius;
     // four emitters per thread for large launches (denser LDS windows)
     const int ept = n >= (1 << 19) ? 4 : 1;
     const int nbb = (int)(((long long)n + (long long)smx::BIN_THREADS * ept - 1) /
                           ((long long)smx::BIN_THREADS * ept));
     const int cs = smx::count_stride(2LL * n, n_tiles);
-    hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                       stream, nframes, frames_dev, n_tiles, ws.tile_count, cs, ws.tile_fov);
-    SMX_CHECK_LAUNCH("tiles_init_kernel");
+    const int tile_blocks = (n_tiles + 1 + 255) / 256;
+    const int table_blocks = (TABLE_ENTRIES + 255) / 256;
+    hipLaunchKernelGGL(gauss_init_kernel, dim3(tile_blocks + table_blocks), dim3(256), 0, stream,
+                       nframes, frames_dev, n_tiles, ws.tile_count, cs, ws.tile_fov, tile_blocks,
+                       P, n, ws.table, ws.hdr);
+    SMX_CHECK_LAUNCH("gauss_init_kernel");
     if (n > 0) {
         if (ept == 4)
             hipLaunchKernelGGL(gauss_bin_count_kernel<4>, dim3(nbb), dim3(smx::BIN_THREADS), 0,
@@ -892,7 +1323,7 @@ int smx_gauss_forward(int n, const float* x, const float* y, const float* z, con
     if (rc) return rc;
     GaussWorkspace ws;
     gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
-    rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, params->radius, capacity, ws,
+    rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, *params, capacity, ws,
                    status_dev, stream);
     if (rc) return rc;
     FwdArgs A;
@@ -921,35 +1352,29 @@ int sep_gemm_run(int n, const float* x, const float* y, const float* z, const fl
     }
     GaussWorkspace ws;
     gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
-    int rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, params->radius, capacity,
+    int rc = gauss_bin(n, x, y, frame_id, frames_dev, nframes, n_tiles, *params, capacity,
                        ws, status_dev, stream);
     if (rc) return rc;
-    if (n > 0) {
-        hipLaunchKernelGGL(gauss_table_kernel, dim3((TABLE_ENTRIES + 255) / 256), dim3(256), 0,
-                           stream, *params, ws.table);
-        SMX_CHECK_LAUNCH("gauss_table_kernel");
-        const int blocks = (int)min(((long long)n + 7) / 8, 4096LL);
-        hipLaunchKernelGGL(sep_factor_kernel, dim3(blocks), dim3(256), 0, stream, n, x, y, z, I,
-                           *params, ws.table, ws.fac, ws.lo);
-        SMX_CHECK_LAUNCH("sep_factor_kernel");
-    }
     SepArgs A;
-    A.fac = ws.fac; A.lo = ws.lo; A.frames = frames_dev; A.nframes = nframes;
+    A.x = x; A.y = y; A.z = z; A.I = I; A.tab = ws.table; A.inv_depth = params->inv_depth;
+    A.R = params->radius; A.W = 2 * params->radius + 1;
+    A.frames = frames_dev; A.nframes = nframes;
     A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
-    A.capacity = capacity; A.W = 2 * params->radius + 1; A.background = params->background;
+    A.capacity = capacity; A.background = params->background;
     A.mu = mu; A.status = status_dev;
     A.k = k; A.grad_mu = grad_mu; A.tile_nll = ws.tile_nll;
     auto launch = [&](auto nll) {
         constexpr bool L = decltype(nll)::value;
-        if (dtype == 0) hipLaunchKernelGGL((sep_gemm_kernel<0, L>), dim3(n_tiles), dim3(64), 0, stream, A);
-        else if (dtype == 1) hipLaunchKernelGGL((sep_gemm_kernel<1, L>), dim3(n_tiles), dim3(64), 0, stream, A);
-        else hipLaunchKernelGGL((sep_gemm_kernel<2, L>), dim3(n_tiles), dim3(64), 0, stream, A);
+        const dim3 grid(n_tiles), block(SG_THREADS);
+        if (dtype == 0) hipLaunchKernelGGL((sep_gemm_kernel<0, L>), grid, block, 0, stream, A);
+        else if (dtype == 1) hipLaunchKernelGGL((sep_gemm_kernel<1, L>), grid, block, 0, stream, A);
+        else hipLaunchKernelGGL((sep_gemm_kernel<2, L>), grid, block, 0, stream, A);
     };
     if (k) launch(std::true_type());
     else launch(std::false_type());
     SMX_CHECK_LAUNCH("sep_gemm_kernel");
     if (k) {
-        hipLaunchKernelGGL(gauss_tile_nll_finish_kernel, dim3((nframes + 63) / 64), dim3(64), 0,
+        hipLaunchKernelGGL(gauss_tile_nll_finish_kernel, dim3((nframes + 3) / 4), dim3(256), 0,
                            stream, frames_dev, nframes, ws.tile_nll, loss);
         SMX_CHECK_LAUNCH("gauss_tile_nll_finish_kernel");
     }
@@ -1027,7 +1452,7 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
                        "bytes)", n, nframes, workspace_bytes);
         return SMX_EVALUE;
     }
-    float4* table = (float4*)((char*)workspace + backward_table_offset(n));
+    float2* table = (float2*)((char*)workspace + backward_table_offset(n));
     if (n == 0) {
         if (gsigma) hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, stream,
                                        (const double*)workspace, 0, 1, gsigma);
@@ -1053,6 +1478,39 @@ int smx_gauss_backward(int n, const float* x, const float* y, const float* z, co
         SMX_CHECK_LAUNCH("sum_f64_kernel");
         hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, stream, (const double*)part,
                            SUM_BLOCKS, SUM_BLOCKS, gsigma);
+        SMX_CHECK_LAUNCH("sum_f64_kernel");
+    }
+    return SMX_OK;
+}
+
+int smx_gauss_backward_binned(int n, const float* x, const float* y, const float* z,
+                              const float* I, const int* frame_id, const smx_fov_t* frames_dev,
+                              int nframes, int n_tiles, const smx_gauss_params_t* params,
+                              long long capacity, const float* grad_mu, float* gx, float* gy,
+                              float* gz, float* gI, double* gsigma, void* workspace,
+                              size_t workspace_bytes, int* status_dev, hipStream_t stream) {
+    if (n < 0 || nframes <= 0 || n_tiles <= 0 || !frames_dev || !grad_mu || !params_ok(params) ||
+        capacity < 0 || (n > 0 && (!x || !y || !z || !I)) || !workspace || !status_dev ||
+        workspace_bytes < gauss_layout(n, n_tiles, capacity, nullptr, nullptr)) {
+        smx::set_error("smx_gauss_backward_binned: invalid arguments (n=%d, nframes=%d, "
+                       "n_tiles=%d, workspace %zu bytes)", n, nframes, n_tiles, workspace_bytes);
+        return SMX_EVALUE;
+    }
+    GaussWorkspace ws;
+    gauss_layout(n, n_tiles, capacity, (char*)workspace, &ws);
+    BwdTileArgs B;
+    B.n = n; B.x = x; B.y = y; B.z = z; B.I = I; B.frame_id = frame_id; B.frames = frames_dev;
+    B.nframes = nframes; B.n_tiles = n_tiles; B.tile_fov = ws.tile_fov;
+    B.tile_begin = ws.tile_begin; B.list = ws.list; B.capacity = capacity; B.P = *params;
+    B.table = ws.table; B.hdr = ws.hdr; B.grad_mu = grad_mu;
+    B.gx = gx; B.gy = gy; B.gz = gz; B.gI = gI; B.tile_gsig = ws.tile_gsig; B.status = status_dev;
+    const int sweep = (int)std::min<long long>(((long long)n + BT_THREADS - 1) / BT_THREADS, 1024LL);
+    hipLaunchKernelGGL(gauss_backward_tile_kernel, dim3(n_tiles + std::max(sweep, 1)),
+                       dim3(BT_THREADS), 0, stream, B);
+    SMX_CHECK_LAUNCH("gauss_backward_tile_kernel");
+    if (gsigma) {
+        hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, stream,
+                           (const double*)ws.tile_gsig, n_tiles, n_tiles, gsigma);
         SMX_CHECK_LAUNCH("sum_f64_kernel");
     }
     return SMX_OK;

@@ -45,34 +45,54 @@ __global__ void compact_counts_kernel(const int* __restrict__ counts, int cs, in
 
 // Exclusive scan of the (strided) tile counters in one workgroup of
 // SCAN_THREADS threads: begin[g] = sum of counts[h * cs] for h < g, g < n.
-// Each thread sums a contiguous run of ceil(n / SCAN_THREADS) counters, the
-// workgroup scans the run totals in LDS, and each thread writes its run's
-// offsets.  Replaces compact + device-wide scan (three launches) up to
-// SCAN_MAX counters.
+// Each thread owns a contiguous run of ceil(n / SCAN_THREADS) counters (held
+// in registers up to SCAN_REG of them, so their loads issue together), the
+// run totals are scanned with wave shuffles plus one pass over the 16 wave
+// totals, and each thread writes its run's offsets.  Replaces compact +
+// device-wide scan (three launches) up to SCAN_MAX counters.
 constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_REG = 8;
 constexpr int SCAN_MAX = 65536;
 __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __restrict__ counts,
                                                                   int cs, int n,
                                                                   int* __restrict__ begin) {
-    __shared__ int s_run[SCAN_THREADS];
-    const int t = threadIdx.x;
+    __shared__ int s_wave[SCAN_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
     const int a = t * per, b = min(n, a + per);
+    const bool reg = per <= SCAN_REG;
+    int c[SCAN_REG];
     int sum = 0;
-    for (int g = a; g < b; ++g) sum += counts[(size_t)g * cs];
-    s_run[t] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan of the run totals
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        const int v = t >= off ? s_run[t - off] : 0;
-        __syncthreads();
-        s_run[t] += v;
-        __syncthreads();
+    if (reg) {
+#pragma unroll
+        for (int r = 0; r < SCAN_REG; ++r) c[r] = a + r < b ? counts[(size_t)(a + r) * cs] : 0;
+#pragma unroll
+        for (int r = 0; r < SCAN_REG; ++r) sum += c[r];
+    } else {
+        for (int g = a; g < b; ++g) sum += counts[(size_t)g * cs];
     }
-    int run = t > 0 ? s_run[t - 1] : 0;
-    for (int g = a; g < b; ++g) {
-        begin[g] = run;
-        run += counts[(size_t)g * cs];
+    int incl = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int run = incl - sum;
+    for (int w = 0; w < wave; ++w) run += s_wave[w];
+    if (reg) {
+#pragma unroll
+        for (int r = 0; r < SCAN_REG; ++r)
+            if (a + r < b) {
+                begin[a + r] = run;
+                run += c[r];
+            }
+    } else {
+        for (int g = a; g < b; ++g) {
+            begin[g] = run;
+            run += counts[(size_t)g * cs];
+        }
     }
 }
 
@@ -81,40 +101,43 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __
 // are staged in LDS (up to TILES_INIT_FOVS of them) and searched by bisection
 // when their offsets ascend, linearly otherwise.
 constexpr int TILES_INIT_FOVS = 2048;
-__global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
-                                                         const smx_fov_t* __restrict__ fovs,
-                                                         int n_tiles, int* __restrict__ tile_count,
-                                                         int cs, int* __restrict__ tile_fov) {
-    __shared__ int s_off[TILES_INIT_FOVS], s_end[TILES_INIT_FOVS];
-    __shared__ int s_unsorted;
+struct TilesInitLds {
+    int off[TILES_INIT_FOVS], end[TILES_INIT_FOVS];
+    int unsorted;
+};
+// body of tiles_init_kernel for workgroup `block` (256 threads)
+__device__ inline void tiles_init_block(TilesInitLds& L, int block, int nfov,
+                                        const smx_fov_t* __restrict__ fovs, int n_tiles,
+                                        int* __restrict__ tile_count, int cs,
+                                        int* __restrict__ tile_fov) {
     const int staged = nfov < TILES_INIT_FOVS ? nfov : TILES_INIT_FOVS;
-    if (threadIdx.x == 0) s_unsorted = 0;
+    if (threadIdx.x == 0) L.unsorted = 0;
     __syncthreads();
     for (int f = threadIdx.x; f < staged; f += blockDim.x) {
-        s_off[f] = fovs[f].tile_off;
-        s_end[f] = fovs[f].tile_off + fovs[f].tiles_r * fovs[f].tiles_c;
+        L.off[f] = fovs[f].tile_off;
+        L.end[f] = fovs[f].tile_off + fovs[f].tiles_r * fovs[f].tiles_c;
     }
     __syncthreads();
     for (int f = threadIdx.x + 1; f < staged; f += blockDim.x)
-        if (s_off[f] < s_end[f - 1]) s_unsorted = 1;
+        if (L.off[f] < L.end[f - 1]) L.unsorted = 1;
     __syncthreads();
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = block * blockDim.x + threadIdx.x;
     if (g > n_tiles) return;
     tile_count[(size_t)g * cs] = 0;
     if (g == n_tiles) return;
     int owner = -1;
-    if (!s_unsorted && staged == nfov) {
-        int lo = 0, hi = nfov;   // last frame with s_off <= g
+    if (!L.unsorted && staged == nfov) {
+        int lo = 0, hi = nfov;   // last frame with off <= g
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (s_off[mid] <= g) lo = mid;
+            if (L.off[mid] <= g) lo = mid;
             else hi = mid;
         }
-        if (nfov > 0 && g >= s_off[lo] && g < s_end[lo]) owner = lo;
+        if (nfov > 0 && g >= L.off[lo] && g < L.end[lo]) owner = lo;
     } else {
         for (int f = 0; f < nfov; ++f) {
-            const int off = f < staged ? s_off[f] : fovs[f].tile_off;
-            const int end = f < staged ? s_end[f] : off + fovs[f].tiles_r * fovs[f].tiles_c;
+            const int off = f < staged ? L.off[f] : fovs[f].tile_off;
+            const int end = f < staged ? L.end[f] : off + fovs[f].tiles_r * fovs[f].tiles_c;
             if (g >= off && g < end) {
                 owner = f;
                 break;
@@ -122,6 +145,14 @@ __global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
         }
     }
     tile_fov[g] = owner;
+}
+
+__global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
+                                                         const smx_fov_t* __restrict__ fovs,
+                                                         int n_tiles, int* __restrict__ tile_count,
+                                                         int cs, int* __restrict__ tile_fov) {
+    __shared__ TilesInitLds L;
+    tiles_init_block(L, blockIdx.x, nfov, fovs, n_tiles, tile_count, cs, tile_fov);
 }
 
 // Ascending bitonic sort of n_pow2 ints in LDS by the whole workgroup.

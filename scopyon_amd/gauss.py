@@ -67,6 +67,7 @@ class GaussianModel:
         self.gsigma = torch.zeros(1, dtype=torch.float64, device=self.device)
         self._ws = None
         self._cap = 0
+        self._bin_key = None   # what the workspace's tile binning was built for
         lib = _lib.lib()
         self._nll_ws = torch.empty(
             max(1, lib.smx_gauss_nll_workspace_bytes(self.nframes, self.max_frame_pixels)),
@@ -130,6 +131,7 @@ class GaussianModel:
         mu = out if out is not None else torch.empty(self.n_pix, dtype=torch.float32,
                                                      device=self.device)
         p = self.params(sigma)
+        self._bin_key = None
         if method == 'lds':
             _lib.call('smx_gauss_forward', n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
                       I.data_ptr(), _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes,
@@ -140,6 +142,7 @@ class GaussianModel:
                       _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes, self.n_tiles,
                       ctypes.byref(p), self.METHODS[method], cap, mu.data_ptr(), ws.data_ptr(),
                       ws.numel(), self.status.data_ptr(), _device.stream())
+        self._bin_key = self._binning_key(x, y, fid, p, n, cap)
         return mu
 
     def forward_nll(self, x, y, z, I, k, frame_id=None, grad_mu=None, sigma=None, mu=None,
@@ -161,11 +164,13 @@ class GaussianModel:
         fid = self._frame_ids(frame_id, n)
         ws, cap = self._workspace(n)
         p = self.params(sigma)
+        self._bin_key = None
         _lib.call('smx_sep_gemm_nll', n, x.data_ptr(), y.data_ptr(), _device.ptr(z), I.data_ptr(),
                   _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes, self.n_tiles,
                   ctypes.byref(p), self.METHODS[method], cap, _device.ptr(mu), k.data_ptr(),
                   _device.ptr(grad_mu), self.loss.data_ptr(), ws.data_ptr(), ws.numel(),
                   self.status.data_ptr(), _device.stream())
+        self._bin_key = self._binning_key(x, y, fid, p, n, cap)
         return self.loss
 
     def nll(self, mu, k, grad_mu=None):
@@ -178,25 +183,57 @@ class GaussianModel:
                   _device.stream())
         return self.loss
 
-    def backward(self, x, y, z, I, grad_mu, frame_id=None, sigma=None, outputs=None):
+    @staticmethod
+    def _binning_key(x, y, fid, p, n, cap):
+        return (x.data_ptr(), x._version, y.data_ptr(), y._version,
+                None if fid is None else (fid.data_ptr(), fid._version), p.sigma, p.radius, n, cap)
+
+    def backward(self, x, y, z, I, grad_mu, frame_id=None, sigma=None, outputs=None,
+                 binned=None):
+        """dL/dx, dL/dy, dL/dz, dL/dI (per emitter) and dL/dsigma (float64
+        device scalar) for grad_mu = dL/dmu.  binned=None reuses the tile
+        binning of this model's last forward when it was built for these
+        very x, y, frame ids and sigma (smx_gauss_backward_binned), and runs
+        the standalone backward otherwise; True requires the binned form,
+        False forces the standalone one.  Both give identical per-emitter
+        gradients."""
         n = x.numel()
         fid = self._frame_ids(frame_id, n)
         self._check(grad_mu, 'grad_mu')
+        for t, nm in ((x, 'x'), (y, 'y'), (z, 'z'), (I, 'I')):
+            self._check(t, nm)
         if outputs is None:
             outputs = [torch.empty(n, dtype=torch.float32, device=self.device) for _ in range(4)]
         gx, gy, gz, gI = outputs
-        ws, _ = self._workspace(n)
+        ws, cap = self._workspace(n)
         p = self.params(sigma)
-        _lib.call('smx_gauss_backward', n, x.data_ptr(), y.data_ptr(), z.data_ptr(), I.data_ptr(),
-                  _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes, ctypes.byref(p),
-                  grad_mu.data_ptr(), gx.data_ptr(), gy.data_ptr(), gz.data_ptr(), gI.data_ptr(),
-                  self.gsigma.data_ptr(), ws.data_ptr(), ws.numel(), _device.stream())
+        reuse = self._bin_key is not None and self._bin_key == self._binning_key(x, y, fid, p, n,
+                                                                                 cap)
+        if binned and not reuse:
+            raise RuntimeError('no forward binning of these emitters and sigma to reuse')
+        if reuse and binned is not False:
+            _lib.call('smx_gauss_backward_binned', n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                      I.data_ptr(), _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes,
+                      self.n_tiles, ctypes.byref(p), cap, grad_mu.data_ptr(), gx.data_ptr(),
+                      gy.data_ptr(), gz.data_ptr(), gI.data_ptr(), self.gsigma.data_ptr(),
+                      ws.data_ptr(), ws.numel(), self.status.data_ptr(), _device.stream())
+        else:
+            # the standalone backward uses the workspace as scratch: the
+            # forward's binning is gone afterwards
+            self._bin_key = None
+            _lib.call('smx_gauss_backward', n, x.data_ptr(), y.data_ptr(), z.data_ptr(),
+                      I.data_ptr(), _device.ptr(fid), self.frames_dev.data_ptr(), self.nframes,
+                      ctypes.byref(p), grad_mu.data_ptr(), gx.data_ptr(), gy.data_ptr(),
+                      gz.data_ptr(), gI.data_ptr(), self.gsigma.data_ptr(), ws.data_ptr(),
+                      ws.numel(), _device.stream())
         return gx, gy, gz, gI, self.gsigma
 
     def check_status(self):
         s = int(self.status.item())
         if s & 2:
             raise RuntimeError('tile list capacity exceeded')
+        if s & 16:
+            raise RuntimeError('binned backward ran on the binning of another forward')
         return s
 
 
