@@ -43,6 +43,15 @@ KERNELS = {
 }
 
 
+def _progress(msg):
+    """One progress line on stderr per finished stage of the benchmark."""
+    print('[bench {:7.1f} s] {}'.format(time.perf_counter() - _T0, msg), file=sys.stderr,
+          flush=True)
+
+
+_T0 = time.perf_counter()
+
+
 def _config(size, exposure):
     import scopyon_amd
     cfg = scopyon_amd.DefaultConfiguration()
@@ -677,20 +686,28 @@ def main():
                      'frac_implementation_bytes': impl / sec / 1e9 / HBM_PEAK_GBS,
                      'launch_ms': kt[dom]},
     }
+    _progress('headline: {:.0f} frames/s, {:.3f} ms per step; stage times {}'.format(
+        batched_fps, ms_step, json.dumps(kt)))
     if 'uneven' in legs:
         result['batched_uneven'] = batched_uneven_leg()
+        _progress('batched_uneven done')
     if 'scale3d' in legs:
         result['scale_3d'] = scale3d_leg()
+        _progress('scale_3d done')
     if 'precision' in legs:
         result['precision'] = precision_leg()
+        _progress('precision done')
     if 'fwd_bwd' in legs:
         result['fwd_bwd'] = fwd_bwd()
         result['fwd_bwd_per_sec'] = result['fwd_bwd']['fwd_bwd_per_sec']
+        _progress('fwd_bwd done: {:.0f} iterations/s'.format(result['fwd_bwd_per_sec']))
     result['max_rel_err_vs_fp64_oracle'] = gpu_vs_oracle_error(S)
+    _progress('error vs the float64 oracle: {:.2e}'.format(result['max_rel_err_vs_fp64_oracle']))
     if not args.no_cpu:
         base = cpu_baseline(S, args.emitters)
         result['cpu_baseline'] = base
         result['gpu_over_cpu'] = batched_fps / base['value']
+        _progress('cpu baseline done')
     print(json.dumps(result))
 
 
