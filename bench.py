@@ -344,13 +344,16 @@ def scale3d_leg(steps=20, warmup=2):
         build = time.perf_counter() - t0
         sec = _replay_rate(sim, steps, warmup)
         entry = {'frames_per_sec': 1.0 / sec, 'ms_per_frame': sec * 1e3, 'table_build_s': build,
-                 'tile_side': sim.tile_side, 'kernel_ms': _kernel_times(sim, 5)}
+                 'tile_side': sim.tile_side, 'quadrant': sim.quadrant,
+                 'kernel_ms': _kernel_times(sim, 5)}
         if mode == 'lut':
             t = sim.tables
             entry.update({'psf': 'radial x depth LUT: {} slices x {} radial bins (float64), '
-                                 '{}^2 phases x {}^2 px float32 pixel integrals, bilinear'.format(
-                                     t.nz, t.nb, t.lut.q + 1, t.m),
-                          'table_bytes': t.K.numel() * 4})
+                                 '{}^2 phases x {}^2 px float32 pixel integrals, bilinear '
+                                 '(splat reads the four taps of a pixel as one 16-byte '
+                                 'cell entry)'.format(t.nz, t.nb, t.lut.q + 1, t.m),
+                          'table_bytes': t.K.numel() * 4,
+                          'cell_table_bytes': t.cells.numel() * 4})
             out['lut'] = entry
         else:
             entry.update({'psf': '1 nm depth keys, float64 summed-area tables (parity)',
@@ -385,7 +388,7 @@ def batched_uneven_leg(steps=20, warmup=3):
             'ms_per_step_10_steps_per_graph': sec10 * 1e3,
             'emitter_frames_per_sec': float(counts.sum()) / sec,
             'status_unsorted_lists': bool(sim.status() & 1),
-            'tile_side': sim.tile_side,
+            'tile_side': sim.tile_side, 'quadrant': sim.quadrant,
             'tile_autotune_ms': {str(k): v for k, v in sim.tile_timings.items()}}
 
 
@@ -666,7 +669,7 @@ def main():
                    'dt_s': dt, 'D_um2_s': 0.1, 'walls': 'reflecting',
                    'emccd': 'Philox4x32-10, float32 sampler' +
                             (', fused into the splat tile pass' if sim.fused else ''),
-                   'tile_side': sim.tile_side,
+                   'tile_side': sim.tile_side, 'quadrant': sim.quadrant,
                    'tile_autotune_ms': {str(k): v for k, v in sim.tile_timings.items()}},
         # plain = the same kernels, one frame per step, eager launches
         'plain_frames_per_sec': plain_fps,

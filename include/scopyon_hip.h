@@ -75,6 +75,14 @@ size_t smx_lut_table_elems(const smx_lut_t* lut /* host */);
 size_t smx_lut_build_workspace_bytes(const smx_lut_t* lut /* host */, int nz_batch);
 int smx_lut_build(const double* radial, int nz, int nb, const smx_lut_t* lut /* host */,
                   double* workspace, size_t workspace_bytes, float* K, hipStream_t stream);
+/* The splat's form of the tables (no reference counterpart): per slice and
+ * cell (cu, cv), cu, cv < q, for each footprint pixel the four phases of
+ * the bilinear taps around it, (cu, cv), (cu, cv + 1), (cu + 1, cv),
+ * (cu + 1, cv + 1), as one 16-byte entry (smx_lut_cell_elems floats per
+ * slice) -- one contiguous load per pixel instead of four. */
+size_t smx_lut_cell_elems(const smx_lut_t* lut /* host */);
+int smx_lut_cells(const float* K, int nz, const smx_lut_t* lut /* host */, float* cells,
+                  hipStream_t stream);
 
 /* ---- emitters and splat (_epifm.py:1267-1333, 215-282) ------------------ */
 /* Per-emitter weight w (the reference's `normalization`) and depth-table key
@@ -127,8 +135,10 @@ int smx_phase_table_fold(const double* K, int nslots, int ntab, int period, int 
  * instead of four); 1: its float32 copy (smx_phase_table_to_float); 2: its
  * folded float32 copy (smx_phase_table_fold).  With 1 and 2 each
  * contribution is rounded to float32.
- * status bit 1: a tile list exceeded the LDS sort capacity (still correct,
- * order not by emitter index); bit 2: list capacity too small (result
+ * status bit 1: a tile list held more than 8192 entries, too many to sort
+ * (still correct, order not by emitter index; shorter lists are sorted in
+ * the tile pass or, past its LDS capacity, after binning); bit 2: list
+ * capacity too small (result
  * incomplete); bit 8: an emitter's table slot was not built (skipped). */
 int smx_splat_expected(int n, const double* u, const double* v, const double* w, const int* key,
                        const int* fov_id, const smx_fov_t* fovs_dev, int nfov, int n_tiles,
@@ -157,9 +167,13 @@ int smx_splat_accumulate(int n, const double* u, const double* v, const double* 
  * phase tables K, 32 without): the FOV descriptors' tiles_r / tiles_c and
  * n_tiles must be counted in tiles of that side.  The plain entry points use
  * 32 (smx_splat_tile_size()).  A tile of side 16 q is processed by q^2 waves,
- * one per 16 x 16 quadrant; the batched path times the three on its own
+ * one per 16 x 16 quadrant; with SMX_TILE_Q32 or-ed into a side of 32 or 64
+ * (phase tables only) one wave takes each 32 x 32 quadrant instead (fewer
+ * footprint pieces per emitter, more LDS per wave).  Binning and results do
+ * not depend on the quadrant.  The batched path times the shapes on its own
  * workload and keeps the fastest.  No reference counterpart (the reference
  * loops over emitters on the whole frame, _epifm.py:215-282). */
+#define SMX_TILE_Q32 0x100
 int smx_splat_expected_ts(int n, const double* u, const double* v, const double* w,
                           const int* key, const int* fov_id, const smx_fov_t* fovs_dev, int nfov,
                           int n_tiles, const double* sat, size_t sat_stride, int n_slots,
@@ -219,8 +233,8 @@ int smx_splat_detect_accumulate_ts(int n, const double* u, const double* v, cons
                                    int tile_side);
 
 /* Pixel integration in LUT mode: the overlay of _epifm.py:215-282 with the
- * pixel integrals of lut_tables (smx_lut_build; key[i] = depth slice of
- * emitter i) instead of 1 nm cell sums.  Emitter i at a = (W/2 + x) / pl
+ * pixel integrals of lut_tables (the cell form, smx_lut_cells; key[i] =
+ * depth slice of emitter i) instead of 1 nm cell sums.  Emitter i at a = (W/2 + x) / pl
  * pixels from the frame edge takes the bilinear combination of the four
  * tabulated phases around a q.  Same tiles, binning, workspace
  * (smx_splat_workspace_bytes), list order and status bits as
@@ -326,8 +340,9 @@ int smx_photon_detector_philox(int n, const double* photons, const double* gain_
 size_t smx_gauss_workspace_bytes(int n, int n_tiles, long long capacity);
 /* mu = background + sum of emitter contributions (every pixel of every frame
  * is written).  capacity: tile-list entries (each emitter enters at most four
- * 32 x 32 tiles).  status bit 1: a tile list was too long to sort in LDS
- * (result correct, summation order not by emitter index); bit 2: capacity too
+ * 32 x 32 tiles).  status bit 1: a tile list held more than 8192 entries,
+ * too many to sort (result correct, summation order not by emitter index);
+ * bit 2: capacity too
  * small (result incomplete). */
 int smx_gauss_forward(int n, const float* x, const float* y, const float* z, const float* I,
                       const int* frame_id, const smx_fov_t* frames_dev, int nframes, int n_tiles,

@@ -143,6 +143,8 @@ _LUT = ctypes.POINTER(LutT)
 SIGNATURES['smx_lut_table_elems'] = (c_size_t, [_LUT])
 SIGNATURES['smx_lut_build_workspace_bytes'] = (c_size_t, [_LUT, c_int])
 SIGNATURES['smx_lut_build'] = (c_int, [P, c_int, c_int, _LUT, P, c_size_t, P, P])
+SIGNATURES['smx_lut_cell_elems'] = (c_size_t, [_LUT])
+SIGNATURES['smx_lut_cells'] = (c_int, [P, c_int, _LUT, P, P])
 _LUT_SPLAT = [c_int, P, P, P, P, P, P, c_int, c_int, P, _LUT, c_longlong, c_int, P, P, c_size_t,
               P, P]
 SIGNATURES['smx_splat_lut_ts'] = (c_int, _LUT_SPLAT + [c_int, c_int])

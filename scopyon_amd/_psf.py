@@ -196,7 +196,8 @@ class LUTTables:
     takes one slice), and per slice the float32 pixel integrals of the
     reference's cartesian kernel (the profile clamped at r_max on the square
     |x|, |y| <= r_max, _epifm.py:98-126) at (q + 1)^2 sub-pixel phases
-    (smx_lut_build).  Emitters take the nearest slice of their 1 nm depth key
+    (smx_lut_build, `K`), and their cell form for the splat (`cells`,
+    smx_lut_cells: per pixel the four phases around each of q^2 cells).  Emitters take the nearest slice of their 1 nm depth key
     and the bilinear combination of the four phases around their position.
 
     Compared with the reference's 1 nm depth keys and 1 nm cell sums this is
@@ -250,6 +251,11 @@ class LUTTables:
         _lib.call('smx_lut_build', self.radial.data_ptr(), self.nz, self.nb, self.lut,
                   ws.data_ptr(), ws.numel(), self.K.data_ptr(), st)
         del ws
+        # the splat's form: the four bilinear taps of a pixel in one 16-byte entry
+        self.cell_elems = int(lib.smx_lut_cell_elems(self.lut))
+        self.cells = torch.empty(self.nz * self.cell_elems, dtype=torch.float32,
+                                 device=self.device)
+        _lib.call('smx_lut_cells', self.K.data_ptr(), self.nz, self.lut, self.cells.data_ptr(), st)
         # key index -> slice: 0..n_keys-2 are 1 nm depth keys, n_keys-1 the cutoff key
         self.n_keys = int(round(depth_cutoff / RES)) + 3
         self.cutoff_index = self.n_keys - 1

@@ -14,11 +14,11 @@ Layout in HBM (structure of arrays): emitter coordinates ``depth, u, v``
 ``pix_off`` offsets: photons/expectation (float64) and ADC counts (float32 by
 default).  The PSF summed-area tables are shared by all FOVs.
 
-The splat works on square tiles of 16, 32 or 64 pixels (one wave per 16 x 16
-quadrant); with ``tile_side='auto'`` (the default when the pixel is an integer
-number of PSF cells) the constructor times the three on the batch it was
-given and keeps the fastest (``tile_timings``, ``autotune_tiles()``).  Frames
-do not depend on the tile side.
+The splat works on square tiles of 16, 32 or 64 pixels, one wave per 16 x 16
+or 32 x 32 quadrant; with ``tile_side='auto'`` (the default when the pixel is
+an integer number of PSF cells, or in LUT mode) the constructor times the five
+shapes on the batch it was given and keeps the fastest (``tile_timings``,
+``autotune_tiles()``).  Frames do not depend on the tile shape.
 """
 import os
 
@@ -37,7 +37,8 @@ class BatchedSimulator:
     def __init__(self, config, fov_shapes, n_emitters, *, D=0.1e-12, dt=None, seed=0,
                  boundary='reflect', box=None, depth_range=None, method='default',
                  adc_dtype=torch.float32, keep_expectation=False, table_precision='folded',
-                 tile_side='auto', fuse_detector=True, keep_photons=False, psf='table',
+                 tile_side='auto', quadrant=16, fuse_detector=True, keep_photons=False,
+                 psf='table',
                  lut_slices=256, lut_bins=4096, lut_phases=32):
         self.device = _device.require_gpu()
         conf = config[method] if hasattr(config, '__getitem__') else config
@@ -57,6 +58,7 @@ class BatchedSimulator:
         p0 = c.detector_focal_point
         self.fov_shapes = [(int(nw), int(nh)) for nw, nh in fov_shapes]
         self.tile_side = 32
+        self.quadrant = 16
         self._set_geometry(32)
         geoms = self.geoms
         if boundary not in ('none', 'periodic', 'reflect'):
@@ -173,7 +175,7 @@ class BatchedSimulator:
         self.k_format = 0
         self.Ktab = None
         if self.lut:
-            self.Ktab = self.tables.K
+            self.Ktab = self.tables.cells
         if self.use_phase:
             self.k_format = {'float64': 0, 'float32': 1, 'folded': 2}[table_precision]
             if self.k_format == 0:
@@ -194,14 +196,29 @@ class BatchedSimulator:
         if tile_side == 'auto':
             if self.use_phase or self.lut:
                 self.autotune_tiles()
-        elif int(tile_side) != 32:
-            if int(tile_side) not in self.TILE_SIDES or not (self.use_phase or self.lut):
-                raise ValueError('tile_side must be 16, 32 or 64 (32 without phase tables)')
-            self._set_geometry(int(tile_side))
-            self._set_workspace()
+        else:
+            shape = (int(tile_side), int(quadrant))
+            if shape != (32, 16):
+                if shape not in self.TILE_SHAPES or not (self.use_phase or self.lut):
+                    raise ValueError('tile_side must be 16, 32 or 64 with quadrant 16, or 32 or '
+                                     '64 with quadrant 32 (32 / 16 without phase tables)')
+                self.quadrant = shape[1]
+                self._set_geometry(shape[0])
+                self._set_workspace()
 
     # ---- tiles ----
     TILE_SIDES = (16, 32, 64)
+    # (tile side, quadrant side): q^2 waves per tile, one per quadrant
+    TILE_SHAPES = ((16, 16), (32, 16), (64, 16), (32, 32), (64, 32))
+
+    @staticmethod
+    def shape_name(side, quadrant):
+        return str(side) if quadrant == 16 else '{}/q{}'.format(side, quadrant)
+
+    def _tile_arg(self):
+        """The C ABI's tile argument: the side, with SMX_TILE_Q32 for 32 x 32
+        quadrants."""
+        return self.tile_side | (0x100 if self.quadrant == 32 else 0)
 
     def _set_geometry(self, ts):
         """FOV descriptors counted in tiles of side ts."""
@@ -226,33 +243,39 @@ class BatchedSimulator:
             dtype=torch.uint8, device=self.device)
 
     def autotune_tiles(self, reps=3):
-        """Pick the splat tile side (16, 32 or 64 pixels: 1, 4 or 16 waves per
-        tile) by timing the binning + tile pass of this batch's current
-        positions with HIP events; a side whose tile lists overflow the sort
-        capacity (status bit 1) is taken only if every side does.  Returns
-        the side kept; the timings (ms) stay in `tile_timings`."""
+        """Pick the splat tile shape -- side 16, 32 or 64 pixels with one wave
+        per 16 x 16 quadrant, or side 32 or 64 with one wave per 32 x 32
+        quadrant -- by timing what a step runs on this batch's current
+        positions (binning + tile pass, with the detector epilogue when it is
+        fused) with HIP events; a shape whose tile lists are too long to sort
+        (status bit 1) is taken only if every shape has such lists.  Returns
+        the (side, quadrant) kept; the timings (ms) stay in `tile_timings`,
+        keyed by shape_name()."""
         if self.graph is not None:
             raise RuntimeError('autotune_tiles() before capture()')
         self.weights()
+        run = self.splat_detect if self.fused else self.splat
         results = {}
-        for ts in self.TILE_SIDES:
-            self._set_geometry(ts)
+        for shape in self.TILE_SHAPES:
+            self.quadrant = shape[1]
+            self._set_geometry(shape[0])
             self._set_workspace()
             self.splatter.status.zero_()
-            self.splat()                           # warm (code objects, caches)
+            run()                                  # warm (code objects, caches)
             torch.cuda.synchronize()
             unsorted = bool(int(self.splatter.status.item()) & 1)
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
             for _ in range(reps):
-                self.splat()
+                run()
             b.record()
             torch.cuda.synchronize()
-            results[ts] = (unsorted, a.elapsed_time(b) / reps)
+            results[shape] = (unsorted, a.elapsed_time(b) / reps)
         self.splatter.status.zero_()
-        self.tile_timings = {ts: ms for ts, (_, ms) in results.items()}
-        best = min(results, key=lambda ts: results[ts])   # sorted lists first, then time
-        self._set_geometry(best)
+        self.tile_timings = {self.shape_name(*k): ms for k, (_, ms) in results.items()}
+        best = min(results, key=lambda k: results[k])   # sorted lists first, then time
+        self.quadrant = best[1]
+        self._set_geometry(best[0])
         self._set_workspace()
         return best
 
@@ -287,10 +310,10 @@ class BatchedSimulator:
             stages = {'smx_splat_expected': 3, 'smx_splat_bin': 1, 'smx_splat_accumulate': 2}
             _lib.call('smx_splat_lut_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
                       self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
-                      self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.K.data_ptr(), t.lut,
+                      self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.cells.data_ptr(), t.lut,
                       self.capacity, 0, self.photons.data_ptr(), self.ws.data_ptr(),
                       self.ws.numel(), self.splatter.status.data_ptr(), _device.stream(),
-                      stages[entry], self.tile_side)
+                      stages[entry], self._tile_arg())
             return
         _lib.call(entry + '_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
@@ -298,7 +321,7 @@ class BatchedSimulator:
                   t.sat_elems, len(t.slot_of), t.ntab, t.period, 1e-9, 1e-18, self.capacity, 0,
                   _device.ptr(self.Ktab), self.k_format, t.m, self.photons.data_ptr(),
                   self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(),
-                  _device.stream(), self.tile_side)
+                  _device.stream(), self._tile_arg())
 
     def detect(self):
         """EMCCD + ADC of every frame, then advance the frame counter."""
@@ -322,12 +345,12 @@ class BatchedSimulator:
             stages = {'smx_splat_detect_ts': 3, 'smx_splat_detect_accumulate_ts': 2}
             _lib.call('smx_splat_lut_detect_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
                       self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
-                      self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.K.data_ptr(), t.lut,
+                      self.fovs_dev.data_ptr(), self.nfov, self.n_tiles, t.cells.data_ptr(), t.lut,
                       self.capacity, 0, out, self.ws.data_ptr(), self.ws.numel(),
                       self.splatter.status.data_ptr(), _device.stream(),
                       _device.ptr(self.gain_map), _device.ptr(self.offset_map), self.det,
                       self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
-                      self.adc.data_ptr(), f32, stages[entry], self.tile_side)
+                      self.adc.data_ptr(), f32, stages[entry], self._tile_arg())
             return
         _lib.call(entry, self.n, self.u.data_ptr(), self.v.data_ptr(), self.w.data_ptr(),
                   self.key.data_ptr(), self.fov_id.data_ptr(), self.fovs_dev.data_ptr(),
@@ -337,7 +360,7 @@ class BatchedSimulator:
                   self.splatter.status.data_ptr(), _device.stream(), _device.ptr(self.gain_map),
                   _device.ptr(self.offset_map), self.det, self.seed, 0, 0,
                   self.counter.data_ptr(), _device.ptr(self.E), self.adc.data_ptr(), f32,
-                  self.tile_side)
+                  self._tile_arg())
 
     def stages(self):
         """The launches of one step, in order, as callables (for timing)."""
@@ -432,7 +455,8 @@ class BatchedSimulator:
         """Status bits of the splats since construction: raises on a tile
         list overflow (bit 2) or an emitter that met an unbuilt PSF table
         (bit 8, its light would be missing); returns the word otherwise (bit
-        1: a tile list too long to sort, summed out of emitter order)."""
+        1: a tile list of more than 8192 entries, summed out of emitter
+        order)."""
         s = int(self.splatter.status.item())
         if s & 2:
             raise RuntimeError('splat tile list capacity exceeded')

@@ -146,6 +146,12 @@ template <typename T>
 __device__ inline T gload(const T* p) {
     return *(const __attribute__((address_space(1))) T*)(p);
 }
+// 16-byte form (p 16-byte aligned): one global_load_dwordx4.
+__device__ inline float4 gload4(const float* p) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = *(const __attribute__((address_space(1))) f4*)(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 
 // Summed-area-table index for a (side x side) corner grid stored either
 // row-major (period == 1) or phase-major with the given period: corners whose
@@ -227,6 +233,18 @@ SMX_HD PhaseFold make_phase_fold(int ntab, int P) {
     f.nc = f.n1 + ((f.c0 + P) / 2 - f.c0);
     f.mf = f.npix(0);
     return f;
+}
+
+// Workgroup -> tile map that keeps neighbouring tiles on one XCD.  The
+// dispatcher hands workgroup b to XCD b % NXCD, so XCD x takes the contiguous
+// range of tiles [x per + min(x, rem), (x + 1) per + min(x + 1, rem)) of the
+// n tiles (per = n / NXCD, rem = n % NXCD): the emitters, records and table
+// blocks that neighbouring tiles share are then fetched into one XCD's L2
+// instead of several.  A bijection of [0, n) for every n.
+constexpr int NXCD = 8;
+__device__ inline int xcd_tile(int b, int n) {
+    const int x = b % NXCD, i = b / NXCD, per = n / NXCD, rem = n % NXCD;
+    return x * per + min(x, rem) + i;
 }
 
 }  // namespace smx

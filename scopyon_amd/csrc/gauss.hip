@@ -183,10 +183,12 @@ __global__ void __launch_bounds__(256) gauss_init_kernel(int nfov,
                                                          int cs, int* __restrict__ tile_fov,
                                                          int tile_blocks, smx_gauss_params_t P,
                                                          int n, float2* __restrict__ tab,
-                                                         GaussHeader* __restrict__ hdr) {
+                                                         GaussHeader* __restrict__ hdr,
+                                                         int* __restrict__ long_tiles) {
     __shared__ smx::TilesInitLds L;
     if ((int)blockIdx.x < tile_blocks) {
-        smx::tiles_init_block(L, blockIdx.x, nfov, fovs, n_tiles, tile_count, cs, tile_fov);
+        smx::tiles_init_block(L, blockIdx.x, nfov, fovs, n_tiles, tile_count, cs, tile_fov,
+                              long_tiles);
         return;
     }
     const int idx = ((int)blockIdx.x - tile_blocks) * blockDim.x + threadIdx.x;
@@ -301,6 +303,11 @@ __global__ void __launch_bounds__(smx::BIN_THREADS) gauss_bin_fill_kernel(
     }
 }
 
+// Whole frames per XCD (smx::xcd_tile) when there are at least as many
+// frames as XCDs: a frame's tiles share emitters and dL/dmu halos in one L2
+// (64 frames of 256^2: binned backward 0.068 -> 0.061 ms).
+static int xcd_map(int nframes) { return nframes >= smx::NXCD ? 1 : 0; }
+
 struct FwdArgs {
     const float* x;
     const float* y;
@@ -315,6 +322,7 @@ struct FwdArgs {
     smx_gauss_params_t P;
     float* mu;
     int* status;
+    int xcd;             // 1: workgroups map to tiles by smx::xcd_tile
 };
 
 __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
@@ -327,7 +335,7 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
     __shared__ int4 s_qi[4][FCHUNK];           // per wave: rectangle in the quadrant
     __shared__ int s_part[4][FCHUNK * 4];      // per wave: emitter << 2 | part
 
-    const int tile = blockIdx.x;
+    const int tile = A.xcd ? smx::xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int fr = A.tile_fov[tile];
     if (fr < 0 || fr >= A.nframes) return;
     const smx_fov_t F = A.frames[fr];
@@ -347,7 +355,10 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 1 && count <= WAVE_SORT_MAX) {
+    if (count >= 1 && count <= 256) {
+        // up to one entry per thread: rank sort by the workgroup
+        smx::rank_sort(list, count, s_sort + 256, s_sort);
+    } else if (count >= 1 && count <= WAVE_SORT_MAX) {
         if (wave == 0) wave_sort_list(list, count, lane, s_sort);
         __syncthreads();
     } else if (count >= 1 && sorted) {
@@ -356,8 +367,6 @@ __global__ void __launch_bounds__(256) gauss_forward_kernel(FwdArgs A) {
         for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
         __syncthreads();
         lds_bitonic_sort(s_sort, np2);
-    } else if (!sorted && threadIdx.x == 0) {
-        atomicOr(A.status, 1);
     }
     // the sorted list goes back in place for the binned backward
     if (sorted && count > 1)
@@ -643,6 +652,7 @@ struct BwdTileArgs {
     float* gI;
     double* tile_gsig;
     int* status;
+    int xcd;             // 1: workgroups map to tiles by smx::xcd_tile
 };
 
 __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTileArgs A) {
@@ -653,7 +663,9 @@ __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTile
     __shared__ int s_own[BT_THREADS];
     __shared__ double s_gs[BT_THREADS];
     // per emitter of a round, window row k: G, dG/dx, dG/dsigma
-    __shared__ float s_rowf[BT_GROUPS][GB][3];
+    // (a float4 per row: one 16-byte read; one padding entry per group puts
+    // the 16 groups of a wave on distinct banks)
+    __shared__ float4 s_rowf[BT_GROUPS][GB + 1];
     __shared__ int s_wcnt[BT_THREADS / 64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int R = A.P.radius, W = 2 * R + 1;
@@ -673,7 +685,7 @@ __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTile
         }
         return;
     }
-    const int tile = blockIdx.x;
+    const int tile = A.xcd ? smx::xcd_tile(blockIdx.x, A.n_tiles) : (int)blockIdx.x;
     const int fr = A.tile_fov[tile];
     const int begin = A.tile_begin[tile];
     int count = A.tile_begin[tile + 1] - begin;
@@ -692,13 +704,23 @@ __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTile
     const int HS = TS + 2 * R;
     const int hr0 = tr * TS - R, hc0 = tc * TS - R;
     // staged rows r < HS + 2 R: frame row hr0 - R + r (guard rows zero),
-    // one row per wave and iteration, one column per lane
-    for (int r = wave; r < HS + 2 * R; r += BT_THREADS / 64) {
+    // one row per wave and step, one column per lane; every load of a
+    // thread is issued before the first store, so the staging waits for one
+    // memory latency instead of one per row
+    constexpr int SROWS = (HALO_MAX + 14 + BT_THREADS / 64 - 1) / (BT_THREADS / 64);
+    float gv[SROWS];
+#pragma unroll
+    for (int i = 0; i < SROWS; ++i) {
+        const int r = wave + i * (BT_THREADS / 64);
         const int pr = hr0 - R + r, pc = hc0 + lane;
-        if (lane < HS) {
-            const bool ok = r >= R && r < HS + R && pr >= 0 && pr < F.nw && pc >= 0 && pc < F.nh;
-            s_g[r * HS + lane] = ok ? A.grad_mu[(size_t)F.pix_off + (size_t)pr * F.nh + pc] : 0.0f;
-        }
+        const bool ok = lane < HS && r >= R && r < HS + R && pr >= 0 && pr < F.nw && pc >= 0 &&
+                        pc < F.nh;
+        gv[i] = ok ? A.grad_mu[(size_t)F.pix_off + (size_t)pr * F.nh + pc] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < SROWS; ++i) {
+        const int r = wave + i * (BT_THREADS / 64);
+        if (lane < HS && r < HS + 2 * R) s_g[r * HS + lane] = gv[i];
     }
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     const int* list = A.list + begin;
@@ -746,22 +768,26 @@ __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTile
             float fu, fv;
             const int su = table_slot(v.x - fxl, fu) + 4 * q, sv = table_slot(v.y - fyl, fv) + 4 * q;
             float gy[4], dgy[4], sgy[4];
+            float ru[3][4];
             int hcol[4];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float4* tu = tab4 + ((c * TABLE_ENTRIES + su) >> 1);
                 const float4* tv = tab4 + ((c * TABLE_ENTRIES + sv) >> 1);
                 const float4 u0 = tu[0], u1 = tu[1], v0 = tv[0], v1 = tv[1];
-                const float ru[4] = {fmaf(fu, u0.y, u0.x), fmaf(fu, u0.w, u0.z),
-                                     fmaf(fu, u1.y, u1.x), fmaf(fu, u1.w, u1.z)};
+                ru[c][0] = fmaf(fu, u0.y, u0.x);
+                ru[c][1] = fmaf(fu, u0.w, u0.z);
+                ru[c][2] = fmaf(fu, u1.y, u1.x);
+                ru[c][3] = fmaf(fu, u1.w, u1.z);
                 const float rv[4] = {fmaf(fv, v0.y, v0.x), fmaf(fv, v0.w, v0.z),
                                      fmaf(fv, v1.y, v1.x), fmaf(fv, v1.w, v1.z)};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (4 * q + i < W) s_rowf[grp][4 * q + i][c] = ru[i];
-                    (c == 0 ? gy : c == 1 ? dgy : sgy)[i] = rv[i];
-                }
+                for (int i = 0; i < 4; ++i) (c == 0 ? gy : c == 1 ? dgy : sgy)[i] = rv[i];
             }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * q + i < W)
+                    s_rowf[grp][4 * q + i] = make_float4(ru[0][i], ru[1][i], ru[2][i], 0.0f);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 // a column outside the staged block (off the frame) or past
@@ -784,8 +810,8 @@ __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTile
 #pragma unroll
             for (int k = 0; k < WMAX; ++k) {
                 if (k < W) {
-                    const float f0 = s_rowf[grp][k][0], f1 = s_rowf[grp][k][1];
-                    const float f2 = s_rowf[grp][k][2];
+                    const float4 fk = s_rowf[grp][k];
+                    const float f0 = fk.x, f1 = fk.y, f2 = fk.z;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const float g = grow[k * HS + hcol[i]];
@@ -872,8 +898,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int SG_WAVES = 4;
 constexpr int SG_THREADS = 64 * SG_WAVES;
 constexpr int SG_STAGE = SG_THREADS;
-constexpr int SG_SORT_CAP = PSORT_CAP;   // tile lists sorted in LDS (longer: status bit 1)
-constexpr int SG_RANK_MAX = 64;          // lists rank-sorted by one wave (longer: bitonic)
+constexpr int SG_SORT_CAP = PSORT_CAP;   // tile lists sorted in LDS (longer: after binning)
+constexpr int SG_RANK_MAX = SG_THREADS;  // lists rank-sorted by the workgroup (longer: bitonic)
 // Factor rows of a chunk emitter: [0] = 0, [1 .. W] = amp * G along the rows,
 // [W + 1 ..] = 0, then the same 18 slots along the columns; an MFMA operand
 // is row[clamp(pixel - window origin, -1, W) + 1], zero outside the window
@@ -902,6 +928,7 @@ struct SepArgs {
     const float* k;      // NLL: observed counts per pixel
     float* grad_mu;      // NLL: dL/dmu = 1 - k / mu (may be NULL)
     double* tile_nll;    // NLL: sum over the tile of mu - k log(mu)
+    int xcd;             // 1: workgroups map to tiles by smx::xcd_tile
 };
 
 struct SepLds {
@@ -917,27 +944,6 @@ struct SepLds {
     double nll[SG_WAVES];
 };
 
-// Rank sort of a tile list of 2 <= count <= SG_SORT_CAP distinct entries by
-// the whole workgroup: every entry's rank is the number of smaller entries
-// (broadcast LDS reads, no dependent shuffle stages).  raw[] must hold count
-// rounded up to 4; out[rank] = entry.
-__device__ inline void rank_sort(const int* __restrict__ list, int count, int* raw, int* out) {
-    const int c4 = (count + 3) & ~3;
-    for (int k = threadIdx.x; k < c4; k += blockDim.x) raw[k] = k < count ? list[k] : 0x7fffffff;
-    __syncthreads();
-    const int4* raw4 = reinterpret_cast<const int4*>(raw);
-    for (int p = threadIdx.x; p < count; p += blockDim.x) {
-        const int v = raw[p];
-        int r = 0;
-        for (int q = 0; q < (c4 >> 2); ++q) {
-            const int4 w = raw4[q];
-            r += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
-        }
-        out[r] = v;
-    }
-    __syncthreads();
-}
-
 template <typename T> struct Frag8 { typedef T type __attribute__((ext_vector_type(8))); };
 
 // DT: operand type, 0: float32 (32x32x2), 1: float16 (32x32x16), 2: bfloat16
@@ -948,7 +954,7 @@ template <int DT, bool NLL>
 __global__ void __launch_bounds__(SG_THREADS) __attribute__((amdgpu_waves_per_eu(6, 8)))
 sep_gemm_kernel(SepArgs A) {
     __shared__ SepLds L;
-    const int tile = blockIdx.x;
+    const int tile = A.xcd ? smx::xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     // independent loads first: frame, list bounds
     const int fr = A.tile_fov[tile];
     const int begin = A.tile_begin[tile];
@@ -979,7 +985,7 @@ sep_gemm_kernel(SepArgs A) {
         // short lists: rank sort by every thread; up to WAVE_SORT_MAX: one
         // wave's register bitonic sort; longer: bitonic sort in LDS
         if (count <= SG_RANK_MAX) {
-            rank_sort(list, count, L.u.raw, L.sort);
+            smx::rank_sort(list, count, L.u.raw, L.sort);
         } else if (count <= WAVE_SORT_MAX) {
             if (wave == 0) wave_sort_list(list, count, l, L.sort);
             __syncthreads();
@@ -995,8 +1001,6 @@ sep_gemm_kernel(SepArgs A) {
     } else if (count == 1) {
         if (tid == 0) L.sort[0] = list[0];
         __syncthreads();
-    } else if (!sorted && tid == 0) {
-        atomicOr(A.status, 1);
     }
 
     f32x16 acc;
@@ -1164,6 +1168,7 @@ struct GaussWorkspace {
     GaussHeader* hdr;    // what the binning and table were built for
     double* tile_nll;    // n_tiles: fused NLL per tile
     double* tile_gsig;   // n_tiles: binned backward's dL/dsigma per tile
+    int* long_tiles;     // n_tiles + 1: tiles with lists too long to sort in LDS
     void* cub_tmp;
     size_t cub_bytes;
 };
@@ -1180,6 +1185,7 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
     size_t o_hdr = take(sizeof(GaussHeader));
     size_t o_tnll = take(sizeof(double) * (size_t)n_tiles);
     size_t o_tgs = take(sizeof(double) * (size_t)n_tiles);
+    size_t o_long = take(sizeof(int) * (size_t)(n_tiles + 1));
     size_t cb = 0;
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, cb, (int*)nullptr, (int*)nullptr, n_tiles + 1);
     size_t o_cub = take(cb);
@@ -1194,6 +1200,7 @@ size_t gauss_layout(int n, int n_tiles, long long capacity, char* base, GaussWor
         ws->hdr = (GaussHeader*)(base + o_hdr);
         ws->tile_nll = (double*)(base + o_tnll);
         ws->tile_gsig = (double*)(base + o_tgs);
+        ws->long_tiles = (int*)(base + o_long);
         ws->cub_tmp = base + o_cub;
         ws->cub_bytes = cb;
     }
@@ -1242,12 +1249,16 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
     const int ept = n >= (1 << 19) ? 4 : 1;
     const int nbb = (int)(((long long)n + (long long)smx::BIN_THREADS * ept - 1) /
                           ((long long)smx::BIN_THREADS * ept));
-    const int cs = smx::count_stride(2LL * n, n_tiles);
+    // padded counters only when a tile holds hundreds of entries: a 13 px
+    // window enters ~2 tiles, and the single-workgroup scan reads padded
+    // counters one cache line each (64 frames of 256^2 with 5,000 emitters:
+    // ~150 entries per tile, dense counters)
+    const int cs = smx::count_stride(2LL * n, n_tiles, 256);
     const int tile_blocks = (n_tiles + 1 + 255) / 256;
     const int table_blocks = (TABLE_ENTRIES + 255) / 256;
     hipLaunchKernelGGL(gauss_init_kernel, dim3(tile_blocks + table_blocks), dim3(256), 0, stream,
                        nframes, frames_dev, n_tiles, ws.tile_count, cs, ws.tile_fov, tile_blocks,
-                       P, n, ws.table, ws.hdr);
+                       P, n, ws.table, ws.hdr, ws.long_tiles);
     SMX_CHECK_LAUNCH("gauss_init_kernel");
     if (n > 0) {
         if (ept == 4)
@@ -1263,12 +1274,14 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
     if (n_tiles + 1 <= smx::SCAN_MAX) {
         // one workgroup scans the counters in place of compact + device scan
         hipLaunchKernelGGL(smx::scan_counts_kernel, dim3(1), dim3(smx::SCAN_THREADS), 0, stream,
-                           ws.tile_count, cs, n_tiles + 1, ws.tile_begin);
+                           ws.tile_count, cs, n_tiles + 1, ws.tile_begin, PSORT_CAP,
+                           ws.long_tiles);
         SMX_CHECK_LAUNCH("scan_counts_kernel");
     } else {
         size_t cb = ws.cub_bytes;
         hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256),
-                           0, stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
+                           0, stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense, PSORT_CAP,
+                           ws.long_tiles);
         SMX_CHECK_LAUNCH("compact_counts_kernel");
         if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
                                                             ws.tile_begin, n_tiles + 1, stream),
@@ -1285,6 +1298,10 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
                                stream, n, x, y, frame_id, frames_dev, nframes, R, n_tiles,
                                ws.tile_begin, ws.tile_count, cs, ws.list, capacity, status_dev);
         SMX_CHECK_LAUNCH("gauss_bin_fill_kernel");
+        hipLaunchKernelGGL(smx::sort_long_lists_kernel, dim3(smx::LONG_SORT_BLOCKS),
+                           dim3(smx::LONG_SORT_THREADS), 0, stream, ws.long_tiles, ws.tile_begin,
+                           ws.list, capacity, status_dev);
+        SMX_CHECK_LAUNCH("sort_long_lists_kernel");
     }
     return SMX_OK;
 }
@@ -1329,7 +1346,7 @@ int smx_gauss_forward(int n, const float* x, const float* y, const float* z, con
     FwdArgs A;
     A.x = x; A.y = y; A.z = z; A.I = I; A.frames = frames_dev; A.nframes = nframes;
     A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
-    A.capacity = capacity; A.P = *params; A.mu = mu; A.status = status_dev;
+    A.capacity = capacity; A.P = *params; A.mu = mu; A.status = status_dev; A.xcd = xcd_map(nframes);
     hipLaunchKernelGGL(gauss_forward_kernel, dim3(n_tiles), dim3(256), 0, stream, A);
     SMX_CHECK_LAUNCH("gauss_forward_kernel");
     return SMX_OK;
@@ -1362,7 +1379,7 @@ int sep_gemm_run(int n, const float* x, const float* y, const float* z, const fl
     A.tile_fov = ws.tile_fov; A.tile_begin = ws.tile_begin; A.list = ws.list;
     A.capacity = capacity; A.background = params->background;
     A.mu = mu; A.status = status_dev;
-    A.k = k; A.grad_mu = grad_mu; A.tile_nll = ws.tile_nll;
+    A.k = k; A.grad_mu = grad_mu; A.tile_nll = ws.tile_nll; A.xcd = xcd_map(nframes);
     auto launch = [&](auto nll) {
         constexpr bool L = decltype(nll)::value;
         const dim3 grid(n_tiles), block(SG_THREADS);
@@ -1504,6 +1521,7 @@ int smx_gauss_backward_binned(int n, const float* x, const float* y, const float
     B.tile_begin = ws.tile_begin; B.list = ws.list; B.capacity = capacity; B.P = *params;
     B.table = ws.table; B.hdr = ws.hdr; B.grad_mu = grad_mu;
     B.gx = gx; B.gy = gy; B.gz = gz; B.gI = gI; B.tile_gsig = ws.tile_gsig; B.status = status_dev;
+    B.xcd = xcd_map(nframes);
     const int sweep = (int)std::min<long long>(((long long)n + BT_THREADS - 1) / BT_THREADS, 1024LL);
     hipLaunchKernelGGL(gauss_backward_tile_kernel, dim3(n_tiles + std::max(sweep, 1)),
                        dim3(BT_THREADS), 0, stream, B);

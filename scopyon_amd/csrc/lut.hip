@@ -105,6 +105,26 @@ __global__ void lut_phase_kernel(const double* __restrict__ sat, FineGrid G, int
     K[idx] = (float)(sum * (G.h * G.h));
 }
 
+// Cell-interleaved copy for the splat: cell (cu, cv), cu, cv < q, holds per
+// footprint pixel the four phases around it, in the order of the bilinear
+// taps: (cu, cv), (cu, cv + 1), (cu + 1, cv), (cu + 1, cv + 1).  One emitter
+// then reads one contiguous 16-byte entry per pixel.
+__global__ void lut_cells_kernel(const float* __restrict__ K, int q, int m, int nz,
+                                 float4* __restrict__ K4) {
+    const size_t mm = (size_t)m * m;
+    const size_t per = (size_t)q * q * mm;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= per * nz) return;
+    const size_t pix = idx % mm;
+    size_t r = idx / mm;
+    const int cv = (int)(r % q); r /= q;
+    const int cu = (int)(r % q);
+    const size_t zi = r / q;
+    const float* Kz = K + zi * (size_t)(q + 1) * (q + 1) * mm;
+    auto at = [&](int a, int b) { return Kz[((size_t)a * (q + 1) + b) * mm + pix]; };
+    K4[idx] = make_float4(at(cu, cv), at(cu, cv + 1), at(cu + 1, cv), at(cu + 1, cv + 1));
+}
+
 bool lut_ok(const smx_lut_t* L) {
     return L && L->q >= 1 && L->q <= 256 && L->ns >= 1 && L->c >= 1 && L->m == 2 * L->c + 1 &&
            L->nslices >= 1 && L->pixel_length > 0.0 && L->r_max > 0.0 &&
@@ -131,6 +151,24 @@ extern "C" {
 size_t smx_lut_table_elems(const smx_lut_t* lut) {
     if (!lut_ok(lut)) return 0;
     return (size_t)(lut->q + 1) * (lut->q + 1) * lut->m * lut->m;
+}
+
+size_t smx_lut_cell_elems(const smx_lut_t* lut) {
+    if (!lut_ok(lut)) return 0;
+    return (size_t)lut->q * lut->q * lut->m * lut->m * 4;
+}
+
+int smx_lut_cells(const float* K, int nz, const smx_lut_t* lut, float* cells, hipStream_t stream) {
+    if (!lut_ok(lut) || nz < 0 || !K || !cells) {
+        smx::set_error("smx_lut_cells: invalid arguments (nz=%d)", nz);
+        return SMX_EVALUE;
+    }
+    const size_t total = smx_lut_cell_elems(lut) / 4 * (size_t)nz;
+    if (total == 0) return SMX_OK;
+    hipLaunchKernelGGL(lut_cells_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, K, lut->q, lut->m, nz, (float4*)cells);
+    SMX_CHECK_LAUNCH("lut_cells_kernel");
+    return SMX_OK;
 }
 
 size_t smx_lut_build_workspace_bytes(const smx_lut_t* lut, int nz_batch) {

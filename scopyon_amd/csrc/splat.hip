@@ -21,6 +21,7 @@
 #include <type_traits>
 #include "common.hpp"
 #include "emccd_block.hpp"
+#include "scopyon_hip.h"
 #include "scopyon_hip_types.h"
 #include "tiles.hpp"
 
@@ -194,8 +195,9 @@ struct TileArgs {
     double res, ext_half, unit_area;
     int ntab;
     int accumulate;          // 1: out += acc ; 0: out = acc
+    int xcd;                 // 1: workgroups map to tiles by smx::xcd_tile
     double* out;
-    int* status;             // set to 1 if a tile list had to be processed unsorted
+    int* status;             // status word (bits as smx_splat_expected documents)
 };
 
 __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
     __shared__ int s_lo[CHUNK][2], s_n[CHUNK][2];   // valid tile-edge range per axis
     __shared__ int s_off[CHUNK + 1];                 // prefix of valid corners per emitter
 
-    const int tile = blockIdx.x;
+    const int tile = A.xcd ? smx::xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int fov = A.tile_fov[tile];
     if (fov < 0 || fov >= A.nfov) return;   // tile not covered by any FOV
     const smx_fov_t F = A.fovs[fov];
@@ -239,8 +241,6 @@ __global__ void __launch_bounds__(256) splat_tile_kernel(TileArgs A) {
         lds_bitonic_sort(u.list, np2);
         for (int k = threadIdx.x; k < count; k += blockDim.x) list[k] = u.list[k];
         __syncthreads();
-    } else if (count > SORT_CAP && threadIdx.x == 0) {
-        atomicOr(A.status, 1);
     }
 
     double acc[PY];
@@ -535,8 +535,8 @@ __device__ inline EmitterRec make_rec_lut(int i, int f, const double* __restrict
     EmitterRec r;
     r.lo_u = r.lo_v = 0;
     r.n_u = r.n_v = 0;
-    r.su = (short)LR.m;
-    r.sv = 1;
+    r.su = (short)(LR.m * 4);   // cell form: 4 floats per pixel
+    r.sv = 4;
     r.flags = 0;
     r.frac = 0;
     r.w = w[i];
@@ -568,7 +568,7 @@ __device__ inline EmitterRec make_rec_lut(int i, int f, const double* __restrict
     }
     r.frac = fr[0] | (fr[1] << 16);
     r.table_off = (long long)slot * (long long)LR.slice_elems +
-                  ((long long)ph[0] * (LR.q + 1) + ph[1]) * (long long)(LR.m * LR.m);
+                  ((long long)ph[0] * LR.q + ph[1]) * (long long)(LR.m * LR.m * 4);
     return r;
 }
 
@@ -730,9 +730,9 @@ struct FastArgs {
     double res, ext_half, unit_area;
     int ntab;
     int accumulate;
+    int xcd;                 // 1: workgroups map to tiles by smx::xcd_tile
     double* out;
     int* status;
-    int tap_u, tap_v;        // LUT mode: table bytes to the next phase along u, v
 };
 
 // Detector epilogue of the fused tile pass (EMCCD + readout noise + ADC of
@@ -787,16 +787,13 @@ __device__ __forceinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& 
 // part touch distinct pixels.  KT is the phase-table element type: double
 // (values identical to the SAT path) or float (each contribution rounded to
 // float32, relative error <= 2^-24).
-constexpr int QS = 16;            // quadrant side
-// Accumulator row stride (doubles).  ds_add_f64 serves 32 lanes per 256 bytes
-// of banks; with 16 the rows of a narrow part share banks, 17 spreads them
-// (measured: 17 is the fastest of 16, 17, 18, 20 and 24 for 32-pixel tiles).
-#ifndef SMX_ACC_STRIDE
-#define SMX_ACC_STRIDE 17
-#endif
-constexpr int AS = SMX_ACC_STRIDE;
+// Quadrant side QS (one wave each): 16, or 32.  Accumulator row stride QS + 1
+// (doubles).  ds_add_f64 serves 32 lanes per 256 bytes of banks; with 16 the
+// rows of a narrow part share banks, 17 spreads them (measured: 17 is the
+// fastest of 16, 17, 18, 20 and 24 for 16 x 16 quadrants).
+template <int QS> constexpr int acc_stride() { return QS + 1; }
 constexpr int NSLOT = 8;          // 64-pixel parts in flight per wave
-// tile list sorted in LDS: 2048 entries, 1024 for one-wave (16 x 16) tiles
+// tile list sorted in LDS: 2048 entries, 1024 for one-wave tiles
 template <int QPW> constexpr int psort_cap() { return QPW == 1 ? 1024 : 2048; }
 
 // One 64-pixel part of an emitter's footprint rectangle inside a quadrant,
@@ -826,14 +823,17 @@ constexpr int PMAX = 64;          // parts per wave per round (one lane writes e
 // buffer once the accumulators are read: from offset 0 for tiles of up to
 // 32 x 32 pixels (one round), after the accumulators for 64 x 64 tiles
 // (four rounds of 1024 pixels that read the accumulators in turn).
-template <int QPW, bool DET, int TAPS>
+template <int QPW, bool DET, int TAPS, int QS>
 struct SplatLds {
+    static constexpr int AS = acc_stride<QS>();
     static constexpr int WPT = QPW * QPW, TSZ = QS * QPW, ACC = QS * AS;
     // accumulators, then one spill slot per lane shared by the waves (lanes
     // outside a part add there: no branch per part, the slots are never read);
     // before the accumulators are cleared the same space sorts the tile list
     static constexpr int ACCN = WPT * ACC + 64 > psort_cap<QPW>() / 2 ? WPT * ACC + 64
                                                                        : psort_cap<QPW>() / 2;
+    // the rank sort's input and output (one entry per thread each) fit too
+    static_assert(WPT == 1 || 2 * 64 * WPT <= 2 * ACCN, "rank sort LDS");
     static constexpr int GEO = (ACCN * 8 + 15) & ~15;   // int4 lo_u, lo_v (frame pixels), n_u, n_v
     static constexpr int STR = GEO + FCHUNK * 16;       // int2 table strides per pixel along u, v
     static constexpr int FLAGS = STR + FCHUNK * 8;
@@ -845,9 +845,10 @@ struct SplatLds {
                                                         // float4 tap coefficients (TAPS 4)
     static constexpr int FRAC = PW + WPT * PMAX * (TAPS == 4 ? 16 : 8);   // LUT fractions
     static constexpr int SPLAT_END = FRAC + (TAPS == 4 ? FCHUNK * 4 : 0);
-    // detector epilogue: rounds of RP pixels, PPT per thread
+    // detector epilogue: rounds of RP pixels, PPT per thread (at most 4)
     static constexpr int NT = 64 * WPT;
-    static constexpr int RP = TSZ * TSZ < 1024 ? TSZ * TSZ : 1024;
+    static constexpr int RPMAX = 4 * NT < 1024 ? 4 * NT : 1024;
+    static constexpr int RP = TSZ * TSZ < RPMAX ? TSZ * TSZ : RPMAX;
     static constexpr int PPT = RP / NT;
     static constexpr int ROUNDS = TSZ * TSZ / RP;
     static constexpr int QOFF = ROUNDS == 1 ? 0 : GEO;
@@ -855,13 +856,14 @@ struct SplatLds {
     static constexpr int BYTES = DET && Q_END > SPLAT_END ? Q_END : SPLAT_END;
 };
 
-template <typename KT, int QPW, bool O32, bool DET, int TAPS>
+template <typename KT, int QPW, bool O32, bool DET, int TAPS, int QS>
 __global__ void __launch_bounds__(64 * QPW * QPW) __attribute__((amdgpu_waves_per_eu(SMX_SPLAT_WPE, 8)))
 splat_phase_kernel(FastArgs A, DetectArgs D) {
-    using LP = SplatLds<QPW, DET, TAPS>;
-    // 64-pixel parts in flight per group: NSLOT table loads, or 4 parts of 4
-    // taps each in LUT mode
-    constexpr int NS = TAPS == 4 ? 4 : NSLOT;
+    using LP = SplatLds<QPW, DET, TAPS, QS>;
+    constexpr int AS = LP::AS;
+    // 64-pixel parts in flight per group: NSLOT table loads (one 16-byte cell
+    // entry per part in LUT mode)
+    constexpr int NS = NSLOT;
     constexpr int WPT = LP::WPT, TSZ = LP::TSZ, PSORT_CAP = psort_cap<QPW>();
     constexpr int ACC = LP::ACC, ACCN = LP::ACCN;
     __shared__ __align__(16) char s_lds[LP::BYTES];
@@ -879,7 +881,7 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
     unsigned* const s_frac = (unsigned*)(s_lds + LP::FRAC);
     (void)ACCN;
 
-    const int tile = blockIdx.x;
+    const int tile = A.xcd ? smx::xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int fov = A.tile_fov[tile];
     if (fov < 0 || fov >= A.nfov) return;
     const smx_fov_t F = A.fovs[fov];
@@ -900,7 +902,12 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    if (count >= 2 && count <= WAVE_SORT_MAX) {
+    // up to one entry per thread (blocks of several waves): rank sort by the
+    // whole block, its input staged after the output
+    constexpr int RANK_MAX = WPT > 1 ? 64 * WPT : 0;
+    if (count >= 2 && count <= RANK_MAX) {
+        smx::rank_sort(list, count, s_sort + RANK_MAX, s_sort);
+    } else if (count >= 2 && count <= WAVE_SORT_MAX) {
         // list of up to 1024 entries: one wave sorts it in registers
         if (wave == 0) wave_sort_list(list, count, lane, s_sort);
         __syncthreads();
@@ -910,8 +917,6 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
         for (int k = threadIdx.x; k < np2; k += blockDim.x) s_sort[k] = k < count ? list[k] : 0x7fffffff;
         __syncthreads();
         lds_bitonic_sort(s_sort, np2);
-    } else if (!sorted && threadIdx.x == 0) {
-        atomicOr(A.status, 1);
     }
     if (count >= 2 && sorted) {
         // sorted list back in place (read per chunk below), then the space
@@ -1050,7 +1055,8 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     for (int b = 0; b < cntp; b += NS) {
-                        KT kv[NS][TAPS];
+                        KT kv[NS];
+                        float4 kc[NS];   // LUT cell: the four taps of a pixel
                         double wv[NS];
                         unsigned la[NS];
 #pragma unroll
@@ -1074,13 +1080,8 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                                 kp = (const char*)(((unsigned long long)(unsigned)g.w << 32) |
                                                    (unsigned)h.x) + off;
                             }
-                            kv[s][0] = smx::gload((const KT*)kp);
-                            if (TAPS == 4) {
-                                kv[s][TAPS > 1 ? 1 : 0] = smx::gload((const KT*)(kp + A.tap_v));
-                                kv[s][TAPS > 2 ? 2 : 0] = smx::gload((const KT*)(kp + A.tap_u));
-                                kv[s][TAPS > 3 ? 3 : 0] =
-                                        smx::gload((const KT*)(kp + (A.tap_u + A.tap_v)));
-                            }
+                            if (TAPS == 4) kc[s] = smx::gload4((const float*)kp);
+                            else kv[s] = smx::gload((const KT*)kp);
                             const unsigned l = (unsigned)__mul24(row, AS * 8) + (unsigned)g.z + ((unsigned)col << 3);
                             la[s] = ok ? l : spill;
                         }
@@ -1088,13 +1089,11 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                         for (int s = 0; s < NS; ++s) {
                             if (TAPS == 4) {
                                 const float4 c = pc[b + s];
-                                const float val = ((c.x * (float)kv[s][0] +
-                                                    c.y * (float)kv[s][TAPS > 1 ? 1 : 0]) +
-                                                   c.z * (float)kv[s][TAPS > 2 ? 2 : 0]) +
-                                                  c.w * (float)kv[s][TAPS > 3 ? 3 : 0];
+                                const float val = ((c.x * kc[s].x + c.y * kc[s].y) + c.z * kc[s].z) +
+                                                  c.w * kc[s].w;
                                 lds_add((double*)((char*)s_acc + la[s]), (double)val);
                             } else {
-                                lds_add((double*)((char*)s_acc + la[s]), (double)kv[s][0] * wv[s]);
+                                lds_add((double*)((char*)s_acc + la[s]), (double)kv[s] * wv[s]);
                             }
                         }
                     }
@@ -1145,7 +1144,7 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
             in[j] = row0 + r < F.nw && col0 + c < F.nh;
             o[j] = F.pix_off + (row0 + r) * F.nh + (col0 + c);
             E[j] = 0.0f;
-            const double a = s_acc[((r >> 4) * QPW + (c >> 4)) * ACC + (r & 15) * AS + (c & 15)];
+            const double a = s_acc[((r / QS) * QPW + (c / QS)) * ACC + (r % QS) * AS + (c % QS)];
             if (in[j]) {
                 E[j] = smx::emccd::expectation_f32(a, D.P);
                 if (A.out) __builtin_nontemporal_store(a, A.out + o[j]);
@@ -1181,6 +1180,7 @@ struct SplatWorkspace {
     int* tile_begin;   // n_tiles + 1
     int* tile_fov;     // n_tiles
     int* list;         // capacity
+    int* long_tiles;   // n_tiles + 1: tiles with lists too long to sort in LDS
     void* cub_tmp;
     size_t cub_bytes;
 };
@@ -1200,6 +1200,7 @@ size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, Spla
     size_t o_begin = take(sizeof(int) * (n_tiles + 1));
     size_t o_fov = take(sizeof(int) * n_tiles);
     size_t o_list = take(sizeof(int) * (size_t)(capacity > 0 ? capacity : 1));
+    size_t o_long = take(sizeof(int) * (size_t)(n_tiles + 1));
     size_t cb = scan_tmp_bytes(n_tiles);
     size_t o_cub = take(cb);
     if (ws && base) {
@@ -1209,6 +1210,7 @@ size_t layout_workspace(int n, int n_tiles, long long capacity, char* base, Spla
         ws->tile_begin = (int*)(base + o_begin);
         ws->tile_fov = (int*)(base + o_fov);
         ws->list = (int*)(base + o_list);
+        ws->long_tiles = (int*)(base + o_long);
         ws->cub_tmp = base + o_cub;
         ws->cub_bytes = cb;
     }
@@ -1270,6 +1272,14 @@ int smx_phase_table_build(const double* sat, size_t sat_stride, int nslots, int 
     return SMX_OK;
 }
 
+// Tiles of one frame share emitters and table blocks.  With many frames,
+// whole frames per XCD (smx::xcd_tile) keep that sharing in one L2 (headline
+// batch: tile pass -2%); one large frame does better with the dispatcher's
+// round robin, whose neighbouring tiles run at the same time on all XCDs and
+// meet their shared lines in the memory-side cache (1024^2 LUT frame: 0.63 ms
+// round robin, 0.76 ms banded).
+static int xcd_map(int nfov) { return nfov >= smx::NXCD ? 1 : 0; }
+
 // stages: 1 = bin the emitters to tiles (and, with K, their table records),
 // 2 = accumulate the tiles from the binned workspace, 3 = both.
 static int splat_run(int n, const double* u, const double* v, const double* w, const int* key,
@@ -1290,7 +1300,7 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         LR.q = lut->q;
         LR.m = lut->m;
         LR.c = lut->c;
-        LR.slice_elems = (size_t)(lut->q + 1) * (lut->q + 1) * lut->m * lut->m;
+        LR.slice_elems = (size_t)lut->q * lut->q * lut->m * lut->m * 4;   // cell form
         sat = (const double*)K;    // not read: LUT records are never integrated through a SAT
         sat_stride = 0;
         n_slots = lut->nslices;
@@ -1300,10 +1310,12 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         k_format = 1;
     }
     const smx::PhaseFold PF = smx::make_phase_fold(ntab, sat_period > 0 ? sat_period : 1);
-    const int lts = tile_side == 16 ? 4 : tile_side == 32 ? 5 : tile_side == 64 ? 6 : -1;
-    if (lts < 0 || (!K && lts != 5)) {
-        smx::set_error("smx_splat_expected: tile side %d (16, 32 or 64 with phase tables; "
-                       "32 without)", tile_side);
+    const bool q32 = (tile_side & SMX_TILE_Q32) != 0;
+    const int side = tile_side & ~SMX_TILE_Q32;
+    const int lts = side == 16 ? 4 : side == 32 ? 5 : side == 64 ? 6 : -1;
+    if (lts < 0 || (!K && (lts != 5 || q32)) || (q32 && lts == 4)) {
+        smx::set_error("smx_splat_expected: tile side %d (16, 32 or 64 with phase tables, "
+                       "SMX_TILE_Q32 with 32 or 64; 32 without)", tile_side);
         return SMX_EVALUE;
     }
     if (n < 0 || nfov <= 0 || n_tiles <= 0 || ntab < 1 || sat_period < 1 || !fovs_dev ||
@@ -1326,12 +1338,16 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     const double ext_half = (res * (double)(ntab - 1)) * 0.5;   // signal_width * 0.5
     const int nb = (n + 255) / 256;
     const int nbr = (n + REC_THREADS - 1) / REC_THREADS;
+    // lists the tile pass sorts itself; longer ones are sorted after binning
+    const int long_min = !K ? SORT_CAP : (lts == 4 || (lts == 5 && q32)) ? psort_cap<1>()
+                                                                         : psort_cap<2>();
     // about two tiles per emitter at these footprints
     const int cs = smx::count_stride(2LL * n, n_tiles);
 
     if (stages & 1) {
         hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                           stream, nfov, fovs_dev, n_tiles, ws.tile_count, cs, ws.tile_fov);
+                           stream, nfov, fovs_dev, n_tiles, ws.tile_count, cs, ws.tile_fov,
+                           ws.long_tiles);
         SMX_CHECK_LAUNCH("tiles_init_kernel");
         if (n > 0) {
             if (K) {
@@ -1353,12 +1369,14 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         if (n_tiles + 1 <= smx::SCAN_MAX) {
             // one workgroup scans the counters in place of compact + device scan
             hipLaunchKernelGGL(smx::scan_counts_kernel, dim3(1), dim3(smx::SCAN_THREADS), 0, stream,
-                               ws.tile_count, cs, n_tiles + 1, ws.tile_begin);
+                               ws.tile_count, cs, n_tiles + 1, ws.tile_begin, long_min,
+                               ws.long_tiles);
             SMX_CHECK_LAUNCH("scan_counts_kernel");
         } else {
             size_t cb = ws.cub_bytes;
             hipLaunchKernelGGL(smx::compact_counts_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256),
-                               0, stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense);
+                               0, stream, ws.tile_count, cs, n_tiles + 1, ws.tile_dense, long_min,
+                               ws.long_tiles);
             SMX_CHECK_LAUNCH("compact_counts_kernel");
             if (smx::check_hip(hipcub::DeviceScan::ExclusiveSum(ws.cub_tmp, cb, ws.tile_dense,
                                                                 ws.tile_begin, n_tiles + 1, stream),
@@ -1375,6 +1393,10 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
                                    fov_id, fovs_dev, nfov, ext_half, n_tiles, ws.tile_begin,
                                    ws.tile_count, cs, ws.list, list_capacity, status_dev);
             SMX_CHECK_LAUNCH("bin_fill_kernel");
+            hipLaunchKernelGGL(smx::sort_long_lists_kernel, dim3(smx::LONG_SORT_BLOCKS),
+                               dim3(smx::LONG_SORT_THREADS), 0, stream, ws.long_tiles,
+                               ws.tile_begin, ws.list, list_capacity, status_dev);
+            SMX_CHECK_LAUNCH("sort_long_lists_kernel");
         }
     }
     if (!(stages & 2)) return SMX_OK;
@@ -1386,9 +1408,7 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
         B.capacity = list_capacity; B.rec = ws.rec; B.K = (const double*)K; B.M = m;
         B.sat = sat; B.sat_stride = sat_stride; B.L = smx::make_sat_layout(ntab + 1, sat_period);
         B.res = res; B.ext_half = ext_half; B.unit_area = unit_area; B.ntab = ntab;
-        B.accumulate = accumulate; B.out = out; B.status = status_dev;
-        B.tap_u = LR.q ? (LR.q + 1) * LR.m * LR.m * 4 : 0;
-        B.tap_v = LR.q ? LR.m * LR.m * 4 : 0;
+        B.accumulate = accumulate; B.out = out; B.status = status_dev; B.xcd = xcd_map(nfov);
         const dim3 grid((unsigned)n_tiles);
         // table byte offsets fit 32 bits: parts address it from its base
         const size_t slot_elems = LR.q ? LR.slice_elems
@@ -1403,12 +1423,17 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
             using KT = decltype(kt);
             constexpr bool O = decltype(o)::value, DT = decltype(dt)::value;
             constexpr int TP = decltype(taps)::value;
+            const bool wide = q32;
             if (lts == 4)
-                hipLaunchKernelGGL((splat_phase_kernel<KT, 1, O, DT, TP>), grid, dim3(64), 0, stream, B, Dv);
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 1, O, DT, TP, 16>), grid, dim3(64), 0, stream, B, Dv);
+            else if (lts == 5 && wide)
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 1, O, DT, TP, 32>), grid, dim3(64), 0, stream, B, Dv);
             else if (lts == 5)
-                hipLaunchKernelGGL((splat_phase_kernel<KT, 2, O, DT, TP>), grid, dim3(256), 0, stream, B, Dv);
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 2, O, DT, TP, 16>), grid, dim3(256), 0, stream, B, Dv);
+            else if (wide)
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 2, O, DT, TP, 32>), grid, dim3(256), 0, stream, B, Dv);
             else
-                hipLaunchKernelGGL((splat_phase_kernel<KT, 4, O, DT, TP>), grid, dim3(1024), 0, stream, B, Dv);
+                hipLaunchKernelGGL((splat_phase_kernel<KT, 4, O, DT, TP, 16>), grid, dim3(1024), 0, stream, B, Dv);
         };
         using One = std::integral_constant<int, 1>;
         using Four = std::integral_constant<int, 4>;
@@ -1435,7 +1460,7 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     A.capacity = list_capacity;
     A.sat = sat; A.sat_stride = sat_stride; A.n_slots = n_slots; A.L = smx::make_sat_layout(ntab + 1, sat_period);
     A.res = res; A.ext_half = ext_half; A.unit_area = unit_area; A.ntab = ntab;
-    A.accumulate = accumulate; A.out = out; A.status = status_dev;
+    A.accumulate = accumulate; A.out = out; A.status = status_dev; A.xcd = xcd_map(nfov);
     hipLaunchKernelGGL(splat_tile_kernel, dim3(n_tiles), dim3(256), 0, stream, A);
     SMX_CHECK_LAUNCH("splat_tile_kernel");
     return SMX_OK;

@@ -32,15 +32,32 @@ __global__ void tile_fov_kernel(int nfov, const smx_fov_t* __restrict__ fovs,
 // 128-byte line): with many emitters per tile the binning atomics serialise
 // on shared lines, with few the padding only spreads the working set.
 constexpr int COUNT_STRIDE = 32;
-inline int count_stride(long long entries, int n_tiles) {
-    return entries >= 128LL * n_tiles ? COUNT_STRIDE : 1;
+inline int count_stride(long long entries, int n_tiles, int dense_per_tile = 128) {
+    return entries >= (long long)dense_per_tile * n_tiles ? COUNT_STRIDE : 1;
+}
+
+// Tile lists longer than a tile pass sorts in LDS (long_min entries) are
+// noted while the counters are scanned: long_tiles[0] = how many,
+// long_tiles[1 ..] = the tiles, in no particular order.  sort_long_lists_kernel
+// then sorts each of them in place before the tile pass, which reads such a
+// list in order from memory.
+__device__ inline void note_long(int* long_tiles, int long_min, int g, int count) {
+    if (long_tiles && count > long_min) {
+        const int k = atomicAdd(long_tiles, 1);
+        long_tiles[1 + k] = g;
+    }
 }
 
 // Dense copy of the counters for the scan: dense[g] = counts[g * cs].
 __global__ void compact_counts_kernel(const int* __restrict__ counts, int cs, int n,
-                                      int* __restrict__ dense) {
+                                      int* __restrict__ dense, int long_min,
+                                      int* __restrict__ long_tiles) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) dense[g] = counts[(size_t)g * cs];
+    if (g < n) {
+        const int c = counts[(size_t)g * cs];
+        dense[g] = c;
+        note_long(long_tiles, long_min, g, c);
+    }
 }
 
 // Exclusive scan of the (strided) tile counters in one workgroup of
@@ -55,7 +72,9 @@ constexpr int SCAN_REG = 8;
 constexpr int SCAN_MAX = 65536;
 __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __restrict__ counts,
                                                                   int cs, int n,
-                                                                  int* __restrict__ begin) {
+                                                                  int* __restrict__ begin,
+                                                                  int long_min,
+                                                                  int* __restrict__ long_tiles) {
     __shared__ int s_wave[SCAN_THREADS / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
@@ -67,9 +86,16 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __
 #pragma unroll
         for (int r = 0; r < SCAN_REG; ++r) c[r] = a + r < b ? counts[(size_t)(a + r) * cs] : 0;
 #pragma unroll
-        for (int r = 0; r < SCAN_REG; ++r) sum += c[r];
+        for (int r = 0; r < SCAN_REG; ++r) {
+            sum += c[r];
+            note_long(long_tiles, long_min, a + r, c[r]);
+        }
     } else {
-        for (int g = a; g < b; ++g) sum += counts[(size_t)g * cs];
+        for (int g = a; g < b; ++g) {
+            const int cg = counts[(size_t)g * cs];
+            sum += cg;
+            note_long(long_tiles, long_min, g, cg);
+        }
     }
     int incl = sum;
 #pragma unroll
@@ -109,7 +135,8 @@ struct TilesInitLds {
 __device__ inline void tiles_init_block(TilesInitLds& L, int block, int nfov,
                                         const smx_fov_t* __restrict__ fovs, int n_tiles,
                                         int* __restrict__ tile_count, int cs,
-                                        int* __restrict__ tile_fov) {
+                                        int* __restrict__ tile_fov,
+                                        int* __restrict__ long_tiles) {
     const int staged = nfov < TILES_INIT_FOVS ? nfov : TILES_INIT_FOVS;
     if (threadIdx.x == 0) L.unsorted = 0;
     __syncthreads();
@@ -124,7 +151,10 @@ __device__ inline void tiles_init_block(TilesInitLds& L, int block, int nfov,
     const int g = block * blockDim.x + threadIdx.x;
     if (g > n_tiles) return;
     tile_count[(size_t)g * cs] = 0;
-    if (g == n_tiles) return;
+    if (g == n_tiles) {
+        if (long_tiles) long_tiles[0] = 0;
+        return;
+    }
     int owner = -1;
     if (!L.unsorted && staged == nfov) {
         int lo = 0, hi = nfov;   // last frame with off <= g
@@ -150,9 +180,10 @@ __device__ inline void tiles_init_block(TilesInitLds& L, int block, int nfov,
 __global__ void __launch_bounds__(256) tiles_init_kernel(int nfov,
                                                          const smx_fov_t* __restrict__ fovs,
                                                          int n_tiles, int* __restrict__ tile_count,
-                                                         int cs, int* __restrict__ tile_fov) {
+                                                         int cs, int* __restrict__ tile_fov,
+                                                         int* __restrict__ long_tiles) {
     __shared__ TilesInitLds L;
-    tiles_init_block(L, blockIdx.x, nfov, fovs, n_tiles, tile_count, cs, tile_fov);
+    tiles_init_block(L, blockIdx.x, nfov, fovs, n_tiles, tile_count, cs, tile_fov, long_tiles);
 }
 
 // Ascending bitonic sort of n_pow2 ints in LDS by the whole workgroup.
@@ -218,6 +249,28 @@ __device__ inline void wave_sort(const int* src, int count, int lane, int* out) 
     for (int r = 0; r < NPL; ++r) out[r * 64 + lane] = v[r];
 }
 
+// Rank sort of a tile list of 2 <= count distinct entries by the whole
+// workgroup: every entry's rank is the number of smaller entries (broadcast
+// LDS reads, no dependent shuffle stages), so a list of up to one entry per
+// thread sorts in one pass over it.  raw[] must hold count rounded up to 4
+// and not overlap out[]; out[rank] = entry.
+__device__ inline void rank_sort(const int* __restrict__ list, int count, int* raw, int* out) {
+    const int c4 = (count + 3) & ~3;
+    for (int k = threadIdx.x; k < c4; k += blockDim.x) raw[k] = k < count ? list[k] : 0x7fffffff;
+    __syncthreads();
+    const int4* raw4 = reinterpret_cast<const int4*>(raw);
+    for (int p = threadIdx.x; p < count; p += blockDim.x) {
+        const int v = raw[p];
+        int r = 0;
+        for (int q = 0; q < (c4 >> 2); ++q) {
+            const int4 w = raw4[q];
+            r += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
+        }
+        out[r] = v;
+    }
+    __syncthreads();
+}
+
 // Sort a tile list of `count` <= WAVE_SORT_MAX entries into out[] by one
 // wave, in registers; out[] must hold the count rounded up to 128, 256, 512
 // or 1024 (the tail is INT_MAX).
@@ -227,6 +280,72 @@ __device__ inline void wave_sort_list(const int* src, int count, int lane, int* 
     else if (count <= 256) wave_sort<4>(src, count, lane, out);
     else if (count <= 512) wave_sort<8>(src, count, lane, out);
     else wave_sort<16>(src, count, lane, out);
+}
+
+// In-place sort of the noted long tile lists (see note_long), up to
+// LONG_SORT_CAP entries each: every wave sorts 1024-entry runs in registers
+// (wave_sort), then the runs are merged pairwise in LDS -- an entry's place in
+// the merged run is its index in its own run plus its rank in the partner run
+// (lower bound from the left run, upper bound from the right one, so the INT_MAX
+// padding of the last run keeps distinct places).  A longer list stays in
+// binning order and sets status bit 1.  Blocks take the noted tiles in a
+// fixed stride; with none noted every block returns at once.
+constexpr int LONG_SORT_THREADS = 512;
+constexpr int LONG_SORT_CAP = 8192;
+constexpr int LONG_SORT_BLOCKS = 256;
+__global__ void __launch_bounds__(LONG_SORT_THREADS) sort_long_lists_kernel(
+        const int* __restrict__ long_tiles, const int* __restrict__ tile_begin,
+        int* __restrict__ list, long long capacity, int* __restrict__ status) {
+    __shared__ int buf[2][LONG_SORT_CAP];
+    const int nlong = long_tiles[0];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int k = blockIdx.x; k < nlong; k += gridDim.x) {
+        const int t = long_tiles[1 + k];
+        const int begin = tile_begin[t];
+        int count = tile_begin[t + 1] - begin;
+        if ((long long)begin + count > capacity) count = (int)max(0LL, capacity - begin);
+        if (count > LONG_SORT_CAP) {
+            if (threadIdx.x == 0) atomicOr(status, 1);
+            continue;
+        }
+        int* L = list + begin;
+        const int nch = (count + 1023) >> 10;
+        for (int c = wave; c < nch; c += LONG_SORT_THREADS / 64)
+            wave_sort<16>(L + c * 1024, min(1024, count - c * 1024), lane, buf[0] + c * 1024);
+        __syncthreads();
+        const int n2 = nch << 10;
+        int src = 0;
+        for (int run = 1024; run < n2; run <<= 1) {
+            const int* a = buf[src];
+            int* out = buf[src ^ 1];
+            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                const int r = i / run, start = r * run;
+                const int pstart = (r ^ 1) * run;
+                const int plen = max(0, min(run, n2 - pstart));
+                const int x = a[i];
+                const int* p = a + pstart;
+                int lo = 0, hi = plen;
+                if ((r & 1) == 0) {
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (p[mid] < x) lo = mid + 1;
+                        else hi = mid;
+                    }
+                } else {
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (p[mid] <= x) lo = mid + 1;
+                        else hi = mid;
+                    }
+                }
+                out[(r & ~1) * run + (i - start) + lo] = x;
+            }
+            __syncthreads();
+            src ^= 1;
+        }
+        for (int i = threadIdx.x; i < count; i += blockDim.x) L[i] = buf[src][i];
+        __syncthreads();
+    }
 }
 
 // Binning with per-block aggregation: a block of BIN_THREADS emitters counts

@@ -64,30 +64,36 @@ def test_many_fovs_span_the_binning_window(gpu):
 
 @pytest.mark.parametrize('precision', ['float64', 'folded'])
 def test_tile_sides_give_identical_frames(gpu, precision):
-    """Tiles of 16, 32 and 64 pixels (1, 4 and 16 waves per tile) sum every
-    pixel's emitters in emitter order, so the frames are bit-identical; the
-    autotuner times all three and keeps one."""
+    """Tiles of 16, 32 and 64 pixels with 16 x 16 quadrants (1, 4 and 16
+    waves per tile) and tiles of 32 and 64 pixels with 32 x 32 quadrants (1
+    and 4 waves) sum every pixel's emitters in emitter order, so the frames
+    are bit-identical; the autotuner times all five and keeps one."""
     from scopyon_amd.batched import BatchedSimulator
     shapes = [(64, 64), (48, 80), (128, 128), (16, 16), (200, 72)]
     counts = [50, 10, 400, 3, 120]
     frames = {}
-    for ts in (16, 32, 64):
+    for ts, q in BatchedSimulator.TILE_SHAPES:
+        print('tile shape {} / {}'.format(ts, q), flush=True)
         sim = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, table_precision=precision,
-                               tile_side=ts)
-        assert sim.tile_side == ts
+                               tile_side=ts, quadrant=q)
+        assert (sim.tile_side, sim.quadrant) == (ts, q)
         sim.form()
         assert sim.status() & ~1 == 0
-        frames[ts] = sim.photons.cpu().numpy()
-    assert frames[32].any()
-    for ts in (16, 64):
-        assert numpy.array_equal(frames[ts].view(numpy.uint64), frames[32].view(numpy.uint64)), ts
+        frames[ts, q] = sim.photons.cpu().numpy()
+    assert frames[32, 16].any()
+    for k in frames:
+        assert numpy.array_equal(frames[k].view(numpy.uint64),
+                                 frames[32, 16].view(numpy.uint64)), k
     auto = BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, table_precision=precision)
-    assert auto.tile_side in (16, 32, 64) and sorted(auto.tile_timings) == [16, 32, 64]
+    assert (auto.tile_side, auto.quadrant) in BatchedSimulator.TILE_SHAPES
+    assert sorted(auto.tile_timings) == sorted(['16', '32', '64', '32/q32', '64/q32'])
     auto.form()
     assert numpy.array_equal(auto.photons.cpu().numpy().view(numpy.uint64),
-                             frames[32].view(numpy.uint64))
+                             frames[32, 16].view(numpy.uint64))
     with pytest.raises(ValueError):
         BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, tile_side=24)
+    with pytest.raises(ValueError):
+        BatchedSimulator(_cfg(), shapes, counts, dt=0.033, seed=3, tile_side=16, quadrant=32)
 
 
 # EmitterRec as the phase-table path stores it at the start of the workspace
@@ -133,25 +139,41 @@ def test_irregular_emitters_match_oracle(gpu, monkeypatch, precision, rtol, tile
 
 def test_long_tile_lists_match_oracle(gpu):
     """6,000 emitters on one 64 x 64 frame give tile lists longer than the
-    in-LDS sort holds (1,024 entries for 16 px tiles, 2,048 for 32 and 64 px):
-    those tiles are processed in list order, not emitter order (status bit 1),
-    and still equal the oracle to rounding (float64 tables)."""
+    tile pass sorts in LDS (1,024 entries for 16 px tiles, 2,048 for 32 and
+    64 px): they are sorted after binning, so every pixel still sums its
+    emitters in index order -- frames equal the oracle to rounding, are
+    bit-identical across tile sides and from run to run, and no status bit
+    is set.  With 10,000 emitters a 64 px tile's list exceeds the 8,192
+    entries that sort holds: status bit 1, values still equal the oracle."""
     from scopyon_amd.batched import BatchedSimulator
-    shape, k = (64, 64), 6000
+    shape = (64, 64)
     p = oracle_params({'detector': {'image_size': list(shape)}, 'magnification': 100.0})
-    ref = None
-    for ts in (16, 32, 64):
-        sim = BatchedSimulator(_cfg(shape), [shape], [k], dt=0.033, seed=17, tile_side=ts,
-                               table_precision='float64')
-        sim.form()
-        assert sim.status() & 1 and sim.status() & ~1 == 0
-        if ref is None:
-            u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
-            parts = numpy.column_stack([numpy.zeros(k), u, v, numpy.arange(k), numpy.ones(k)])
-            ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, shape, numpy.zeros(3), 0.033, {},
-                                         None, None)
-        got = sim.frame(0, 'photons').cpu().numpy()
-        numpy.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * ref.max(), err_msg=str(ts))
+    for k, sides in ((6000, (16, 32, 64)), (10000, (64,))):
+        print('long lists: {} emitters'.format(k), flush=True)
+        ref = first = None
+        for ts in sides:
+            sim = BatchedSimulator(_cfg(shape), [shape], [k], dt=0.033, seed=17, tile_side=ts,
+                                   table_precision='float64')
+            sim.form()
+            if k <= 8192:
+                assert sim.status() == 0
+            else:
+                assert sim.status() & 1 and sim.status() & ~1 == 0
+            if ref is None:
+                u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
+                parts = numpy.column_stack([numpy.zeros(k), u, v, numpy.arange(k), numpy.ones(k)])
+                ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, shape, numpy.zeros(3), 0.033,
+                                             {}, None, None)
+            got = sim.frame(0, 'photons').cpu().numpy()
+            numpy.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * ref.max(),
+                                          err_msg=str(ts))
+            if k <= 8192:
+                if first is None:
+                    first = got
+                assert numpy.array_equal(got.view(numpy.uint64), first.view(numpy.uint64)), ts
+                sim.form()
+                again = sim.frame(0, 'photons').cpu().numpy()
+                assert numpy.array_equal(again.view(numpy.uint64), got.view(numpy.uint64)), ts
 
 
 @pytest.mark.parametrize('precision', ['float64', 'float32', 'folded'])

@@ -23,9 +23,9 @@ def _cfg():
                            'effects': {'photo_bleaching': {'switch': False}}})
 
 
-def _pair(ts, precision='folded', adc_dtype=torch.float32, keep_photons=True, seed=3):
+def _pair(ts, precision='folded', adc_dtype=torch.float32, keep_photons=True, seed=3, q=16):
     from scopyon_amd.batched import BatchedSimulator
-    kw = dict(dt=0.033, seed=seed, table_precision=precision, tile_side=ts,
+    kw = dict(dt=0.033, seed=seed, table_precision=precision, tile_side=ts, quadrant=q,
               adc_dtype=adc_dtype, keep_expectation=True)
     sep = BatchedSimulator(_cfg(), SHAPES, COUNTS, fuse_detector=False, **kw)
     fus = BatchedSimulator(_cfg(), SHAPES, COUNTS, fuse_detector=True, keep_photons=keep_photons,
@@ -39,11 +39,11 @@ def _bits(t):
     return a.view(numpy.uint64 if a.dtype == numpy.float64 else numpy.uint32)
 
 
-@pytest.mark.parametrize('ts', [16, 32, 64])
+@pytest.mark.parametrize('ts,q', [(16, 16), (32, 16), (64, 16), (32, 32), (64, 32)])
 @pytest.mark.parametrize('precision', ['folded', 'float64'])
-def test_fused_equals_separate_passes(gpu, ts, precision):
-    print('fused detector: tile side {} table {}'.format(ts, precision), flush=True)
-    sep, fus = _pair(ts, precision)
+def test_fused_equals_separate_passes(gpu, ts, q, precision):
+    print('fused detector: tile side {} quadrant {} table {}'.format(ts, q, precision), flush=True)
+    sep, fus = _pair(ts, precision, q=q)
     for sim in (sep, fus):
         sim.photons.fill_(-1.0)
         sim.adc.fill_(-1.0)

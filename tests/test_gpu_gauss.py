@@ -379,25 +379,28 @@ def test_large_launch_binning_against_oracle(gpu, method):
 
 
 def test_mfma_forward_deterministic_and_unsorted_lists(gpu):
-    """Repeatable bits for sorted tile lists; a tile list longer than the LDS
-    sort keeps the image (summation order aside) and reports status bit 1."""
+    """Repeatable bits for sorted tile lists, also for lists longer than the
+    tile pass sorts in LDS (2,048 entries: sorted after binning); a list of
+    more than 8,192 entries keeps the image (summation order aside) and
+    reports status bit 1."""
     from scopyon_amd.gauss import GaussianModel
     shapes = [(64, 64)]
     rng = numpy.random.RandomState(7)
     kw = dict(sigma=1.2, inv_depth=0.0, background=1.0, radius=6)
     x, y = rng.uniform(0, 64, 600), rng.uniform(0, 64, 600)
-    I = rng.uniform(1.0, 2.0, len(x))
-    m = GaussianModel(shapes, method='mfma-f32', **kw)
-    args = (_dev(x), _dev(y), None, _dev(I))
-    a = m.forward(*args).clone()
-    b = m.forward(*args).clone()
-    assert torch.equal(a, b)
-    assert m.check_status() == 0
-    x = numpy.concatenate([rng.uniform(8, 24, 2500), x])
-    y = numpy.concatenate([rng.uniform(8, 24, 2500), y])
-    I = rng.uniform(1.0, 2.0, len(x))
-    m = GaussianModel(shapes, method='mfma-f32', **kw)
-    a = m.forward(_dev(x), _dev(y), None, _dev(I))
-    assert m.check_status() & 1
-    ref = G.render(shapes, x, y, numpy.zeros(len(x)), I, numpy.zeros(len(x), int), **kw)[0]
-    numpy.testing.assert_allclose(a.view(64, 64).cpu().numpy(), ref, rtol=2e-5)
+    for extra in (0, 2500, 9000):
+        print('gauss lists: {} extra emitters'.format(extra), flush=True)
+        xs = numpy.concatenate([rng.uniform(8, 24, extra), x])
+        ys = numpy.concatenate([rng.uniform(8, 24, extra), y])
+        I = rng.uniform(1.0, 2.0, len(xs))
+        m = GaussianModel(shapes, method='mfma-f32', **kw)
+        args = (_dev(xs), _dev(ys), None, _dev(I))
+        a = m.forward(*args).clone()
+        if extra <= 2500:
+            b = m.forward(*args).clone()
+            assert torch.equal(a, b)
+            assert m.check_status() == 0
+        else:
+            assert m.check_status() & 1
+        ref = G.render(shapes, xs, ys, numpy.zeros(len(xs)), I, numpy.zeros(len(xs), int), **kw)[0]
+        numpy.testing.assert_allclose(a.view(64, 64).cpu().numpy(), ref, rtol=2e-5)

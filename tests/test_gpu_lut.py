@@ -50,6 +50,14 @@ def test_lut_tables_match_oracle(gpu, oracle):
             sums.append(got.sum())
     # every phase's footprint holds the whole kernel: same photon total
     assert abs(sums[0] - sums[1]) <= 1e-6 * sums[0]
+    # the splat's cell form: per pixel the four phases around cell (cu, cv)
+    # in tap order (cu, cv), (cu, cv + 1), (cu + 1, cv), (cu + 1, cv + 1)
+    q = t.lut.q
+    C = t.cells.view(t.nz, q, q, t.m, t.m, 4)
+    for s, cu, cv in ((0, 0, 0), (77, 31, 31), (255, 12, 30)):
+        got = C[s, cu, cv].cpu().numpy()
+        for k, (a, b) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):
+            assert numpy.array_equal(got[..., k], K[s, cu + a, cv + b].cpu().numpy()), (s, cu, cv, k)
 
 
 def _emitters(sim, rng, depths):
@@ -130,13 +138,13 @@ def test_lut_error_budget(gpu, oracle):
     assert err_p < 3e-2
 
 
-@pytest.mark.parametrize('ts', [16, 32, 64])
-def test_lut_fused_detector_and_tile_sides(gpu, ts):
+@pytest.mark.parametrize('ts,q', [(16, 16), (32, 16), (64, 16), (32, 32), (64, 32)])
+def test_lut_fused_detector_and_tile_sides(gpu, ts, q):
     """Fused LUT splat + EMCCD equals the separate passes bit for bit; frames
-    do not depend on the tile side."""
-    print('lut: fused detector, tile side {}'.format(ts), flush=True)
+    do not depend on the tile shape."""
+    print('lut: fused detector, tile side {} quadrant {}'.format(ts, q), flush=True)
     shapes, counts = [(64, 64), (40, 56)], [60, 30]
-    kw = dict(seed=6, tile_side=ts, keep_expectation=True)
+    kw = dict(seed=6, tile_side=ts, quadrant=q, keep_expectation=True)
     sep = _sim(shapes, counts, fuse_detector=False, **kw)
     fus = _sim(shapes, counts, fuse_detector=True, keep_photons=True, **kw)
     assert fus.fused and not sep.fused
