@@ -1132,23 +1132,27 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
     const uint32_t frame = D.frame + (D.frame_dev ? *D.frame_dev : 0u);
     const uint32_t k0 = (uint32_t)D.seed, k1 = (uint32_t)(D.seed >> 32);
     const int t = threadIdx.x;
-    const long long row0 = (long long)tr * TSZ, col0 = (long long)tc * TSZ;
+    const int row0 = tr * TSZ, col0 = tc * TSZ;
+    // the tile's first pixel in the batch buffer; a pixel's offset from it,
+    // r * nh + c, stays 32-bit (a tile spans at most 64 frame rows)
+    const long long tbase = F.pix_off + (long long)row0 * F.nh + col0;
+    const uint32_t pid0 = D.pix_base + (uint32_t)tbase;
     for (int rd = 0; rd < LP::ROUNDS; ++rd) {
         float E[PPT];
         bool in[PPT];
-        long long o[PPT];
+        int o[PPT];
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             const int p = rd * RP + t + j * NT;
             const int r = p / TSZ, c = p % TSZ;
             in[j] = row0 + r < F.nw && col0 + c < F.nh;
-            o[j] = F.pix_off + (row0 + r) * F.nh + (col0 + c);
+            o[j] = r * F.nh + c;
             E[j] = 0.0f;
             const double a = s_acc[((r / QS) * QPW + (c / QS)) * ACC + (r % QS) * AS + (c % QS)];
             if (in[j]) {
                 E[j] = smx::emccd::expectation_f32(a, D.P);
-                if (A.out) __builtin_nontemporal_store(a, A.out + o[j]);
-                if (D.E_out) D.E_out[o[j]] = (double)E[j];
+                if (A.out) __builtin_nontemporal_store(a, A.out + tbase + o[j]);
+                if (D.E_out) D.E_out[tbase + o[j]] = (double)E[j];
             }
         }
         if (LP::QOFF == 0) __syncthreads();   // accumulators read before the queues overlay them
@@ -1157,18 +1161,19 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                 L, E, in,
                 [&](int loc) {
                     const int p = rd * RP + loc;
-                    return D.pix_base + (uint32_t)(F.pix_off + (row0 + p / TSZ) * F.nh +
-                                                   (col0 + p % TSZ));
+                    return pid0 + (uint32_t)((p / TSZ) * F.nh + (p % TSZ));
                 },
                 frame, k0, k1, D.P, S, noise);
+        float* const adc32 = (float*)D.adc_out + tbase;
+        double* const adc64 = (double*)D.adc_out + tbase;
 #pragma unroll
         for (int j = 0; j < PPT; ++j) {
             if (!in[j]) continue;
-            const float gain = (float)(D.gain_map ? D.gain_map[o[j]] : D.P.gain);
-            const float offset = (float)(D.offset_map ? D.offset_map[o[j]] : D.P.offset);
+            const float gain = (float)(D.gain_map ? D.gain_map[tbase + o[j]] : D.P.gain);
+            const float offset = (float)(D.offset_map ? D.offset_map[tbase + o[j]] : D.P.offset);
             const float x = smx::emccd::adc_f32(S[j], noise[j], D.P, gain, offset);
-            if (D.adc_f32) ((float*)D.adc_out)[o[j]] = x;
-            else ((double*)D.adc_out)[o[j]] = (double)x;
+            if (D.adc_f32) adc32[o[j]] = x;
+            else adc64[o[j]] = (double)x;
         }
     }
 }
