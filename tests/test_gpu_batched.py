@@ -283,6 +283,39 @@ def test_3d_batch_epi_matches_oracle(gpu):
         numpy.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * ref.max())
 
 
+def test_3d_batch_full_depth_range_matches_oracle(gpu):
+    """The 3D scale-up's parity form at its full depth range: +-1 um with 1 nm
+    keys builds every key up to the depth cutoff (1002 float64 summed-area
+    tables, 32 GB); emitters spread over the whole range, the cutoff key
+    included, against the float64 oracle (general summed-area path, EPI)."""
+    from scopyon_amd.batched import BatchedSimulator
+    ov = {'detector': {'image_size': [40, 36], 'exposure_time': 0.01}, 'magnification': 241.0,
+          'light_source': {'angle': 0.0},
+          'effects': {'photo_bleaching': {'switch': False}}}
+    depths = numpy.array([-999.6e-9, -987.4e-9, -512.3e-9, -100.2e-9, 0.7e-9, 333.3e-9,
+                          777.7e-9, 999.9e-9, 1000.4e-9])
+    k = len(depths)
+    print('3D parity tables over +-1 um', flush=True)
+    sim = BatchedSimulator(product_config(ov), [(40, 36)], [k], dt=0.01, seed=6,
+                           depth_range=(-1e-6, 1e-6), D=numpy.full(k, 0.5e-12), psf='table')
+    assert len(sim.tables.slot_of) >= 1002
+    rng = numpy.random.RandomState(8)
+    pl = 16e-6 / 241.0
+    u, v = rng.uniform(-14 * pl, 14 * pl, k), rng.uniform(-12 * pl, 12 * pl, k)
+    sim.set_positions(depths, u, v)
+    sim.form()
+    assert sim.status() & ~1 == 0
+    p = oracle_params({'detector': {'image_size': [40, 36], 'exposure_time': 0.01},
+                       'magnification': 241.0, 'light_source': {'angle': 0.0}})
+    parts = numpy.column_stack([depths, u, v, numpy.arange(k), numpy.ones(k)])
+    ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, (40, 36), numpy.zeros(3), 0.01, {},
+                                 None, None)
+    got = sim.frame(0, 'photons').cpu().numpy()
+    numpy.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * ref.max())
+    del sim
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize('kind', ['CCD', 'CMOS'])
 def test_batched_ccd_cmos_steps(gpu, kind):
     from scopyon_amd.batched import BatchedSimulator
