@@ -287,12 +287,20 @@ int smx_detector_expectation(int n, const double* photons,
 size_t smx_scan_workspace_bytes(int n);
 int smx_exclusive_scan_i32(int n, const int* in, int* out, void* workspace, size_t bytes,
                            hipStream_t stream);
-/* Total pmf mass for one expectation value (written to *T_out, device). */
+/* The pmf of one expectation value shared by many pixels (the background-
+ * only ones): T_out (device, smx_emccd_common_elems() doubles) receives the
+ * total mass and the partial sums smx_emccd_parity searches. */
+size_t smx_emccd_common_elems(void);
 int smx_emccd_common_total(double E, int emgain, double* T_out, hipStream_t stream);
 /* EMCCD.get_signal + ADC with caller-drawn random numbers
  * (_epifm.py:393-416 and 1472-1484): noise[i] = readout noise (NULL: 0),
  * uniforms[uidx[i]] = the uniform RandomState.choice draws for pixel i.
- * Pixels with E == E_common reuse *T_common (may be NULL).  status bit 4:
+ * Pixels with E == E_common reuse T_common (smx_emccd_common_total's output
+ * for E_common; may be NULL).  The draw: the support's terms in chunks of
+ * 512 (8 consecutive terms per lane), the chunk where the cumulative mass
+ * passes u T found from the chunk sums, then the lane and the index inside
+ * it (RandomState.choice's searchsorted(cdf, u, 'right') on the normalised
+ * cumulative sum, equal unless u falls within rounding of a cdf step).  status bit 4:
  * the pmf overflowed for some pixel (the reference raises ValueError). */
 int smx_emccd_parity(int n, const double* E, const double* noise, const double* uniforms,
                      const int* uidx, const double* gain_map, const double* offset_map,

@@ -449,7 +449,8 @@ class _EPIFMSimulator:
                              device=self.device)
         E_common = float(c.detector_qeff * ((0.0 + c.background_mean) if c.background_switch
                                             else 0.0))
-        T_common = _device.ARENA.typed('det_tcommon', 1, torch.float64, self.device)
+        T_common = _device.ARENA.typed('det_tcommon', int(_lib.lib().smx_emccd_common_elems()),
+                                       torch.float64, self.device)
         _lib.call('smx_emccd_common_total', E_common, self.det.emgain, T_common.data_ptr(), st)
         self.splatter.status.zero_()
         _lib.call('smx_emccd_parity', n, E.data_ptr(), _device.ptr(noise), uni.data_ptr(),

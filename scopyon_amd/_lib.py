@@ -88,6 +88,7 @@ SIGNATURES = {
     'smx_detector_expectation': (c_int, [c_int, P, ctypes.POINTER(DetectorParamsT), P, P, P]),
     'smx_scan_workspace_bytes': (c_size_t, [c_int]),
     'smx_exclusive_scan_i32': (c_int, [c_int, P, P, P, c_size_t, P]),
+    'smx_emccd_common_elems': (c_size_t, []),
     'smx_emccd_common_total': (c_int, [c_double, c_int, P, P]),
     'smx_emccd_parity': (c_int, [c_int, P, P, P, P, P, P, ctypes.POINTER(DetectorParamsT),
                                  c_double, P, P, P, P, P]),

@@ -8,8 +8,9 @@
 //  * parity  : the reference's discrete pmf on S in [s_min, s_max) drawn by
 //              inverse CDF with a given uniform per pixel (the uniform numpy's
 //              RandomState.choice would consume).  One wave per pixel: pass 1
-//              sums the pmf, pass 2 scans it until the cumulative mass passes
-//              u * total.  Readout noise comes in pre-drawn.
+//              sums the pmf by chunks, pass 2 finds the chunk and then the
+//              term where the cumulative mass passes u * total.  Readout
+//              noise comes in pre-drawn.
 //  * philox  : statistically equivalent sampler driven by Philox4x32-10:
 //              n ~ Poisson(E), S = round(Gamma(n, emgain)) (S >= 1 when n >= 1),
 //              noise ~ N(0, readout_noise^2).  O(1) per pixel.
@@ -76,11 +77,121 @@ __global__ void prepare_kernel(int n, const double* __restrict__ photons, smx_de
     if (flag) flag[i] = E > 0.0 ? 1 : 0;
 }
 
-// T_common = total pmf mass for one expectation value shared by many pixels.
+// Chunked form of the parity draw: the support's terms in chunks of PCL
+// consecutive S, PJ consecutive terms per lane; pass 1 keeps every chunk's
+// sum (lane sums, then one wave reduction per chunk) and the total is their
+// sum in chunk order, so pass 2 finds the chunk where the cumulative mass
+// passes u T from the chunk sums and evaluates only that chunk's terms (one
+// support plus a chunk instead of about one and a half supports).  Supports
+// of more than PCH chunks take the unchunked passes.
+constexpr int PJ = 8;
+constexpr int PCL = 64 * PJ;
+constexpr int PCH = 192;
+
+__device__ inline void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Pass 1, chunked: ch[c] = sum of terms PCL c .. PCL c + PCL - 1 of the
+// support (c < nch); returns the total, the chunk sums added in order.
+__device__ inline double emccd_chunk_sums(double E, double a, long long s_min, long long len,
+                                          long long nch, int lane, double* ch) {
+    double T = 0.0;
+    for (long long c = 0; c < nch; ++c) {
+        const long long k0 = PCL * c + PJ * lane;
+        double p = 0.0;
+        for (int j = 0; j < PJ; ++j)
+            if (k0 + j < len) p += emccd_term(s_min + k0 + j, E, a);
+        const double v = wave_sum(p);
+        if (lane == 0) ch[c] = v;
+        T += v;
+    }
+    wave_sync_lds();
+    return T;
+}
+
+// Pass 2: the first support index whose cumulative mass exceeds thr (the
+// last index when rounding leaves none): the chunk from the chunk sums, then
+// the lane from a prefix scan of the lanes' sums of the chunk's terms, then
+// the term among that lane's PJ (continuing into the next chunks if the
+// recomputed terms have not yet passed thr at the chunk's end).
+__device__ inline long long emccd_find(double E, double a, long long s_min, long long len,
+                                       long long nch, const double* ch, double thr, int lane) {
+    double carry = 0.0;
+    long long c0 = nch;
+    for (long long b = 0; b < nch; b += 64) {
+        const long long c = b + lane;
+        double v = c < nch ? ch[c] : 0.0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double o = __shfl_up(v, off, 64);
+            if (lane >= off) v += o;
+        }
+        const double cum = carry + v;
+        const double up = __shfl_up(cum, 1, 64);
+        const double before = lane == 0 ? carry : up;   // mass before chunk c
+        const unsigned long long hit = __ballot(c < nch && cum > thr);
+        if (hit) {
+            const int l = __ffsll((long long)hit) - 1;
+            c0 = b + l;
+            carry = __shfl(before, l, 64);
+            break;
+        }
+        carry = __shfl(cum, 63, 64);
+    }
+    if (c0 >= nch) return len - 1;
+    for (long long cb = PCL * c0; cb < len; cb += PCL) {
+        const long long k0 = cb + PJ * lane;
+        double p = 0.0;
+        for (int j = 0; j < PJ; ++j)
+            if (k0 + j < len) p += emccd_term(s_min + k0 + j, E, a);
+        double v = p;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double o = __shfl_up(v, off, 64);
+            if (lane >= off) v += o;
+        }
+        const double cum = carry + v;
+        const double up = __shfl_up(cum, 1, 64);
+        const double before = lane == 0 ? carry : up;   // mass before this lane's terms
+        const unsigned long long hit = __ballot(k0 < len && cum > thr);
+        if (hit) {
+            // lane l's PJ terms, one per lane j < PJ, scanned from the mass
+            // before them; rounding may leave the scan at or below thr at the
+            // end: then lane l's last term
+            const int l = __ffsll((long long)hit) - 1;
+            const long long kl = cb + PJ * l;
+            const double bl = __shfl(before, l, 64);
+            double t = lane < PJ && kl + lane < len ? emccd_term(s_min + kl + lane, E, a) : 0.0;
+#pragma unroll
+            for (int off = 1; off < PJ; off <<= 1) {
+                const double o = __shfl_up(t, off, 64);
+                if (lane >= off) t += o;
+            }
+            const unsigned long long in = __ballot(lane < PJ && bl + t > thr);
+            const long long found = kl + (in ? (long long)(__ffsll((long long)in) - 1) : PJ - 1);
+            return found < len - 1 ? found : len - 1;
+        }
+        carry = __shfl(cum, 63, 64);
+    }
+    return len - 1;
+}
+
+// The pmf of one expectation value shared by many pixels: T_out[0] = total,
+// T_out[1 ..] = its chunk sums (when the support has at most PCH chunks).
 __global__ void emccd_common_total_kernel(double E, int g, double* __restrict__ T_out) {
     int lane = threadIdx.x & 63;
     double a = 1.0 / (double)g;
-    double T = E > 0.0 ? emccd_total(E, g, a, lane) : 0.0;
+    double T = 0.0;
+    if (E > 0.0) {
+        long long s_min, s_max;
+        emccd_support(E, g, s_min, s_max);
+        const long long len = s_max - s_min, nch = (len + PCL - 1) / PCL;
+        if (nch <= PCH) T = emccd_chunk_sums(E, a, s_min, len, nch, lane, T_out + 1);
+        else T = emccd_total(E, g, a, lane);
+    }
     if (lane == 0) *T_out = T;
 }
 
@@ -90,6 +201,7 @@ __global__ void __launch_bounds__(256) emccd_parity_kernel(
         const double* __restrict__ gain_map, const double* __restrict__ offset_map,
         smx_detector_params_t P, double E_common, const double* __restrict__ T_common,
         double* __restrict__ adc_out, double* __restrict__ signal_out, int* __restrict__ status) {
+    __shared__ double s_ch[4][PCH];   // chunk sums, one pixel per wave
     const int lane = threadIdx.x & 63;
     const int pix = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (pix >= n) return;
@@ -104,11 +216,21 @@ __global__ void __launch_bounds__(256) emccd_parity_kernel(
     } else if (E > 0.0) {
         long long s_min, s_max;
         emccd_support(E, g, s_min, s_max);
-        const long long len = s_max - s_min;
-        double T = (T_common && E == E_common) ? *T_common : emccd_total(E, g, a, lane);
+        const long long len = s_max - s_min, nch = (len + PCL - 1) / PCL;
+        const bool common = T_common && E == E_common;
+        const bool chunked = nch <= PCH;
+        double* const ch = s_ch[threadIdx.x >> 6];
+        double T;
+        if (common) T = *T_common;
+        else if (chunked) T = emccd_chunk_sums(E, a, s_min, len, nch, lane, ch);
+        else T = emccd_total(E, g, a, lane);
         if (!(T > 0.0) || !(T < INFINITY)) {
             // the reference's pmf overflows here (numpy.random.choice raises ValueError)
             if (lane == 0) atomicOr(status, 4);
+        } else if (chunked) {
+            const double thr = uniforms[uidx[pix]] * T;
+            S = (double)(s_min + emccd_find(E, a, s_min, len, nch, common ? T_common + 1 : ch,
+                                            thr, lane));
         } else {
             const double thr = uniforms[uidx[pix]] * T;
             double carry = 0.0;
@@ -296,6 +418,8 @@ int smx_exclusive_scan_i32(int n, const int* in, int* out, void* workspace, size
         return SMX_ERUNTIME;
     return SMX_OK;
 }
+
+size_t smx_emccd_common_elems(void) { return 1 + PCH; }
 
 int smx_emccd_common_total(double E, int emgain, double* T_out, hipStream_t stream) {
     if (emgain <= 0 || !T_out) {

@@ -77,6 +77,42 @@ def test_emccd_parity_zero_and_edge_uniforms(gpu):
     assert numpy.array_equal(adc, ref_adc)
 
 
+def test_emccd_parity_common_total_and_long_supports(gpu):
+    """Pixels sharing the background-only expectation search the pmf's chunk
+    sums computed once (smx_emccd_common_total); with EM gain 1000 the
+    supports of E ~ 60-150 exceed the chunk store and take the unchunked
+    passes.  Both against the oracle's RandomState.choice draws."""
+    from scopyon_amd import _lib, _device
+    rng = numpy.random.RandomState(12)
+    for emgain, E_common, others in ((300, 0.0092, [0.3, 4.0, 37.0, 250.0]),
+                                     (1000, 0.92, [0.2, 3.0, 64.0, 150.0])):
+        print('parity common total: emgain {}'.format(emgain), flush=True)
+        p = oracle_params({'detector': {'image_size': [8, 8], 'emgain': emgain}})
+        E = numpy.full((8, 8), E_common)
+        E.flat[[3, 17, 40, 63]] = others
+        noise = rng.normal(0, p.readout_noise, E.shape)
+        u = rng.random_sample(E.size)
+        ref_adc, ref_S = _oracle(p, E, noise, u)
+        det = _det_params(p)
+        n = E.size
+        T = torch.empty(int(_lib.lib().smx_emccd_common_elems()), dtype=torch.float64,
+                        device=gpu)
+        _lib.call('smx_emccd_common_total', E_common, emgain, T.data_ptr(), _device.stream())
+        dE = _device.upload(E.ravel(), device=gpu)
+        d_uidx = _device.upload(numpy.arange(n), dtype=torch.int32, device=gpu)
+        d_u = _device.upload(u, device=gpu)
+        d_noise = _device.upload(noise.ravel(), device=gpu)
+        adc = torch.empty(n, dtype=torch.float64, device=gpu)
+        sig = torch.empty(n, dtype=torch.float64, device=gpu)
+        status = torch.zeros(1, dtype=torch.int32, device=gpu)
+        _lib.call('smx_emccd_parity', n, dE.data_ptr(), d_noise.data_ptr(), d_u.data_ptr(),
+                  d_uidx.data_ptr(), None, None, det, E_common, T.data_ptr(), adc.data_ptr(),
+                  sig.data_ptr(), status.data_ptr(), _device.stream())
+        assert int(status.item()) == 0
+        assert numpy.array_equal(sig.cpu().numpy().reshape(E.shape), ref_S)
+        assert numpy.array_equal(adc.cpu().numpy().reshape(E.shape), ref_adc)
+
+
 def test_emccd_parity_fpn_maps(gpu):
     rng = numpy.random.RandomState(3)
     p = oracle_params({'detector': {'image_size': [16, 16]},
