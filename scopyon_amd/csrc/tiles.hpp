@@ -62,13 +62,14 @@ __global__ void compact_counts_kernel(const int* __restrict__ counts, int cs, in
 
 // Exclusive scan of the (strided) tile counters in one workgroup of
 // SCAN_THREADS threads: begin[g] = sum of counts[h * cs] for h < g, g < n.
-// Each thread owns a contiguous run of ceil(n / SCAN_THREADS) counters (held
-// in registers up to SCAN_REG of them, so their loads issue together), the
+// Each thread owns a contiguous run of ceil(n / SCAN_THREADS) counters, read
+// in batches of SCAN_REG whose loads issue together (one batch for up to
+// 32768 counters: the values stay in registers for the second pass), the
 // run totals are scanned with wave shuffles plus one pass over the 16 wave
 // totals, and each thread writes its run's offsets.  Replaces compact +
 // device-wide scan (three launches) up to SCAN_MAX counters.
 constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_REG = 8;
+constexpr int SCAN_REG = 32;
 constexpr int SCAN_MAX = 65536;
 __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __restrict__ counts,
                                                                   int cs, int n,
@@ -79,22 +80,17 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
     const int a = t * per, b = min(n, a + per);
-    const bool reg = per <= SCAN_REG;
+    const int nb = (per + SCAN_REG - 1) / SCAN_REG;
     int c[SCAN_REG];
     int sum = 0;
-    if (reg) {
+    for (int bi = 0; bi < nb; ++bi) {
+        const int g0 = a + bi * SCAN_REG;
 #pragma unroll
-        for (int r = 0; r < SCAN_REG; ++r) c[r] = a + r < b ? counts[(size_t)(a + r) * cs] : 0;
+        for (int r = 0; r < SCAN_REG; ++r) c[r] = g0 + r < b ? counts[(size_t)(g0 + r) * cs] : 0;
 #pragma unroll
         for (int r = 0; r < SCAN_REG; ++r) {
             sum += c[r];
-            note_long(long_tiles, long_min, a + r, c[r]);
-        }
-    } else {
-        for (int g = a; g < b; ++g) {
-            const int cg = counts[(size_t)g * cs];
-            sum += cg;
-            note_long(long_tiles, long_min, g, cg);
+            if (g0 + r < b) note_long(long_tiles, long_min, g0 + r, c[r]);
         }
     }
     int incl = sum;
@@ -107,18 +103,19 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __
     __syncthreads();
     int run = incl - sum;
     for (int w = 0; w < wave; ++w) run += s_wave[w];
-    if (reg) {
+    for (int bi = 0; bi < nb; ++bi) {
+        const int g0 = a + bi * SCAN_REG;
+        if (nb > 1) {
+#pragma unroll
+            for (int r = 0; r < SCAN_REG; ++r)
+                c[r] = g0 + r < b ? counts[(size_t)(g0 + r) * cs] : 0;
+        }
 #pragma unroll
         for (int r = 0; r < SCAN_REG; ++r)
-            if (a + r < b) {
-                begin[a + r] = run;
+            if (g0 + r < b) {
+                begin[g0 + r] = run;
                 run += c[r];
             }
-    } else {
-        for (int g = a; g < b; ++g) {
-            begin[g] = run;
-            run += counts[(size_t)g * cs];
-        }
     }
 }
 
