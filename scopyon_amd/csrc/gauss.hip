@@ -1249,11 +1249,11 @@ int gauss_bin(int n, const float* x, const float* y, const int* frame_id,
     const int ept = n >= (1 << 19) ? 4 : 1;
     const int nbb = (int)(((long long)n + (long long)smx::BIN_THREADS * ept - 1) /
                           ((long long)smx::BIN_THREADS * ept));
-    // padded counters only when a tile holds hundreds of entries: a 13 px
+    // dense counters when the blocks' LDS windows hold every tile; beyond
+    // that, padded ones only when a tile holds hundreds of entries: a 13 px
     // window enters ~2 tiles, and the single-workgroup scan reads padded
-    // counters one cache line each (64 frames of 256^2 with 5,000 emitters:
-    // ~150 entries per tile, dense counters)
-    const int cs = smx::count_stride(2LL * n, n_tiles, 256);
+    // counters one cache line each
+    const int cs = smx::agg_count_stride(2LL * n, n_tiles, 256);
     const int tile_blocks = (n_tiles + 1 + 255) / 256;
     const int table_blocks = (TABLE_ENTRIES + 255) / 256;
     hipLaunchKernelGGL(gauss_init_kernel, dim3(tile_blocks + table_blocks), dim3(256), 0, stream,

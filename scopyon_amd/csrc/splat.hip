@@ -1346,8 +1346,9 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     // lists the tile pass sorts itself; longer ones are sorted after binning
     const int long_min = !K ? SORT_CAP : (lts == 4 || (lts == 5 && q32)) ? psort_cap<1>()
                                                                          : psort_cap<2>();
-    // about two tiles per emitter at these footprints
-    const int cs = smx::count_stride(2LL * n, n_tiles);
+    // about two tiles per emitter at these footprints; the record path bins
+    // with per-block aggregation, the SAT path with one atomic per entry
+    const int cs = K ? smx::agg_count_stride(2LL * n, n_tiles) : smx::count_stride(2LL * n, n_tiles);
 
     if (stages & 1) {
         hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
