@@ -914,7 +914,7 @@ constexpr int SG_WAVES = 4;
 constexpr int SG_THREADS = 64 * SG_WAVES;
 constexpr int SG_STAGE = SG_THREADS;
 constexpr int SG_SORT_CAP = PSORT_CAP;   // tile lists sorted in LDS (longer: after binning)
-constexpr int SG_RANK_MAX = SG_THREADS;  // lists rank-sorted by the workgroup (longer: bitonic)
+
 // Factor rows of a chunk emitter: [0] = 0, [1 .. W] = amp * G along the rows,
 // [W + 1 ..] = 0, then the same 18 slots along the columns; an MFMA operand
 // is row[clamp(pixel - window origin, -1, W) + 1], zero outside the window
@@ -950,7 +950,7 @@ struct SepArgs {
 struct SepLds {
     int sort[SG_SORT_CAP];                // the tile's list in index order
     union alignas(16) {
-        int raw[SG_RANK_MAX];             // the list as binned (rank sort input)
+        int raw[SG_WAVES * 256];          // sorted runs of the list (merge_rank_sort)
         float red[16 * 64];               // the waves' accumulators, summed in turn
     } u;
     int2 lo[SG_STAGE];                    // window origin (frame pixels)
@@ -998,13 +998,14 @@ sep_gemm_kernel(SepArgs A) {
     int* list = A.list + begin;
     const bool sorted = count <= SG_SORT_CAP;
     if (count >= 2 && sorted) {
-        // short lists: rank sort by every thread; up to WAVE_SORT_MAX: one
-        // wave's register bitonic sort; longer: bitonic sort in LDS
-        if (count <= SG_RANK_MAX) {
-            smx::rank_sort(list, count, L.u.raw, L.sort);
-        } else if (count <= WAVE_SORT_MAX) {
-            if (wave == 0) wave_sort_list(list, count, l, L.sort);
-            __syncthreads();
+        // up to 1024 entries: every wave sorts a run in registers and the
+        // runs are ranked against each other; longer: bitonic sort in LDS
+        if (count <= SG_WAVES * 64) {
+            smx::merge_rank_sort<1>(list, count, L.u.raw, L.sort);
+        } else if (count <= SG_WAVES * 128) {
+            smx::merge_rank_sort<2>(list, count, L.u.raw, L.sort);
+        } else if (count <= SG_WAVES * 256) {
+            smx::merge_rank_sort<4>(list, count, L.u.raw, L.sort);
         } else {
             int np2 = 1;
             while (np2 < count) np2 <<= 1;

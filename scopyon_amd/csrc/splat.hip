@@ -832,8 +832,10 @@ struct SplatLds {
     // before the accumulators are cleared the same space sorts the tile list
     static constexpr int ACCN = WPT * ACC + 64 > psort_cap<QPW>() / 2 ? WPT * ACC + 64
                                                                        : psort_cap<QPW>() / 2;
-    // the rank sort's input and output (one entry per thread each) fit too
-    static_assert(WPT == 1 || 2 * 64 * WPT <= 2 * ACCN, "rank sort LDS");
+    // the merge sort's runs and output (up to 4 entries per thread each, at
+    // most the sort capacity) fit too
+    static_assert(WPT == 1 || 2 * (4 * 64 * WPT < psort_cap<QPW>() ? 4 * 64 * WPT : psort_cap<QPW>()) <= 2 * ACCN,
+                  "merge sort LDS");
     static constexpr int GEO = (ACCN * 8 + 15) & ~15;   // int4 lo_u, lo_v (frame pixels), n_u, n_v
     static constexpr int STR = GEO + FCHUNK * 16;       // int2 table strides per pixel along u, v
     static constexpr int FLAGS = STR + FCHUNK * 8;
@@ -902,11 +904,18 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
     if ((long long)begin + count > A.capacity) count = (int)max(0LL, A.capacity - begin);
     int* list = A.list + begin;
     const bool sorted = count <= PSORT_CAP;
-    // up to one entry per thread (blocks of several waves): rank sort by the
-    // whole block, its input staged after the output
+    // blocks of several waves: up to one entry per thread, rank sort by the
+    // whole block (its input staged after the output); up to 4 per thread,
+    // every wave sorts a run in registers and the runs are ranked against each
+    // other (merge_rank_sort: measured, the rank sort is the faster of the two
+    // for the ~80-entry lists of 32 px tiles, the merge for longer ones)
     constexpr int RANK_MAX = WPT > 1 ? 64 * WPT : 0;
     if (count >= 2 && count <= RANK_MAX) {
         smx::rank_sort(list, count, s_sort + RANK_MAX, s_sort);
+    } else if (count >= 2 && count <= 2 * RANK_MAX) {
+        smx::merge_rank_sort<2>(list, count, s_sort + 2 * RANK_MAX, s_sort);
+    } else if (count >= 2 && count <= 4 * RANK_MAX && count <= PSORT_CAP) {
+        smx::merge_rank_sort<4>(list, count, s_sort + 4 * RANK_MAX, s_sort);
     } else if (count >= 2 && count <= WAVE_SORT_MAX) {
         // list of up to 1024 entries: one wave sorts it in registers
         if (wave == 0) wave_sort_list(list, count, lane, s_sort);

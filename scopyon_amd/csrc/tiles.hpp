@@ -256,6 +256,47 @@ __device__ inline void wave_sort(const int* src, int count, int lane, int* out) 
     for (int r = 0; r < NPL; ++r) out[r * 64 + lane] = v[r];
 }
 
+// Sort of a tile list of 2 <= count <= 64 * NPL * (waves of the workgroup)
+// distinct entries (< INT_MAX) by the whole workgroup: wave w sorts entries
+// w RL .. w RL + RL - 1 (RL = 64 NPL) in registers into runs[w RL ..]
+// (wave_sort), then an entry's place is its index in its own run plus, for
+// every other run, how many of that run's entries are below it (a binary
+// search; the INT_MAX padding never is).  runs[] holds (count rounded up to
+// RL) ints, out[] count ints, not overlapping.  The same permutation as a rank
+// sort, with (runs - 1) log2(2 RL) LDS reads per entry instead of count / 4.
+template <int NPL>
+__device__ inline void merge_rank_sort(const int* __restrict__ list, int count, int* runs,
+                                       int* out) {
+    constexpr int RL = 64 * NPL;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nruns = (count + RL - 1) / RL;
+    const int own = min(RL, count - w * RL);   // entries of this wave's run (<= 0: none)
+    if (w < nruns) wave_sort<NPL>(list + w * RL, own, lane, runs + w * RL);
+    __syncthreads();
+    if (w < nruns) {
+#pragma unroll
+        for (int r = 0; r < NPL; ++r) {
+            const int i = r * 64 + lane;
+            if (i >= own) continue;
+            const int v = runs[w * RL + i];
+            int place = i;
+            for (int o = 0; o < nruns; ++o) {
+                if (o == w) continue;
+                const int* b = runs + o * RL;
+                int below = 0;   // entries of run o below v
+#pragma unroll
+                for (int step = RL; step > 0; step >>= 1) {
+                    const int idx = below + step - 1;
+                    if (idx < RL && b[idx] < v) below += step;
+                }
+                place += below;
+            }
+            out[place] = v;
+        }
+    }
+    __syncthreads();
+}
+
 // Rank sort of a tile list of 2 <= count distinct entries by the whole
 // workgroup: every entry's rank is the number of smaller entries (broadcast
 // LDS reads, no dependent shuffle stages), so a list of up to one entry per
