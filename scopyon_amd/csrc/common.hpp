@@ -153,6 +153,21 @@ __device__ inline float4 gload4(const float* p) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave on DPP lane moves (VALU,
+// no LDS crossbar traffic, unlike __shfl_up's ds_bpermute): shifts by 1, 2,
+// 4 and 8 inside each row of 16 lanes (lanes shifted in from outside the row
+// add 0), then lane 15 of rows 0 and 2 into rows 1 and 3, then lane 31 into
+// rows 2 and 3.  Call with every lane of the wave active.
+__device__ inline int wave_incl_scan(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return v;
+}
+
 // Summed-area-table index for a (side x side) corner grid stored either
 // row-major (period == 1) or phase-major with the given period: corners whose
 // row/column indices share residues mod `period` are contiguous, so one

@@ -975,14 +975,9 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                 }
             }
             const bool any_irregular = __ballot(irr) != 0;
-            int incl = np;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                int t = __shfl_up(incl, d);
-                if (lane >= d) incl += t;
-            }
+            const int incl = smx::wave_incl_scan(np);
             const int first = incl - np;   // this emitter's parts: [first, incl) in list order
-            const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+            const int total = __builtin_amdgcn_readlane(incl, 63);
 
             if (any_irregular) {
                 // rare (a float64 tie in a cell boundary): one part at a time, in

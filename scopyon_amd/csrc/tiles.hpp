@@ -72,60 +72,63 @@ __global__ void compact_counts_kernel(const int* __restrict__ counts, int cs, in
 
 // Exclusive scan of the (strided) tile counters in one workgroup of
 // SCAN_THREADS threads: begin[g] = sum of counts[h * cs] for h < g, g < n.
-// Each thread owns a contiguous run of ceil(n / SCAN_THREADS) counters, read
-// in batches of SCAN_REG whose loads issue together (one batch for up to
-// 32768 counters: the values stay in registers for the second pass), the
-// run totals are scanned with wave shuffles plus one pass over the 16 wave
-// totals, and each thread writes its run's offsets.  Replaces compact +
-// device-wide scan (three launches) up to SCAN_MAX counters.
+// The counters go in rows of SCAN_THREADS (thread t holds counter
+// row * SCAN_THREADS + t: every load and store of a wave is coalesced),
+// SCAN_ROWS rows at a time: the rows' wave scans are independent shuffle
+// chains that overlap, their wave totals meet in LDS behind one barrier, and
+// each row adds the totals of the waves before it and of the rows before it.
+// Replaces compact + device-wide scan (three launches) up to SCAN_MAX
+// counters.  (With a barrier per row, or a run of consecutive counters per
+// thread, 16,385 counters took 13 us on MI355X against 2.3 us for an empty
+// launch.)
 constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_REG = 32;
+constexpr int SCAN_ROWS = 8;
 constexpr int SCAN_MAX = 65536;
 __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __restrict__ counts,
                                                                   int cs, int n,
                                                                   int* __restrict__ begin,
                                                                   int long_min,
                                                                   int* __restrict__ long_tiles) {
-    __shared__ int s_wave[SCAN_THREADS / 64];
+    constexpr int NW = SCAN_THREADS / 64;
+    static_assert(NW == 16, "one DPP row of 16 lanes scans the wave totals");
+    __shared__ int s_wave[SCAN_ROWS][NW];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-    const int a = t * per, b = min(n, a + per);
-    const int nb = (per + SCAN_REG - 1) / SCAN_REG;
-    int c[SCAN_REG];
-    int sum = 0;
-    for (int bi = 0; bi < nb; ++bi) {
-        const int g0 = a + bi * SCAN_REG;
+    int carry = 0;   // sum of the rows before
+    for (int base = 0; base < n; base += SCAN_THREADS * SCAN_ROWS) {
+        int c[SCAN_ROWS], incl[SCAN_ROWS];
 #pragma unroll
-        for (int r = 0; r < SCAN_REG; ++r) c[r] = g0 + r < b ? counts[(size_t)(g0 + r) * cs] : 0;
-#pragma unroll
-        for (int r = 0; r < SCAN_REG; ++r) {
-            sum += c[r];
-            if (g0 + r < b) note_long(long_tiles, long_min, g0 + r, c[r]);
-        }
-    }
-    int incl = sum;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d);
-        if (lane >= d) incl += v;
-    }
-    if (lane == 63) s_wave[wave] = incl;
-    __syncthreads();
-    int run = incl - sum;
-    for (int w = 0; w < wave; ++w) run += s_wave[w];
-    for (int bi = 0; bi < nb; ++bi) {
-        const int g0 = a + bi * SCAN_REG;
-        if (nb > 1) {
-#pragma unroll
-            for (int r = 0; r < SCAN_REG; ++r)
-                c[r] = g0 + r < b ? counts[(size_t)(g0 + r) * cs] : 0;
+        for (int r = 0; r < SCAN_ROWS; ++r) {
+            const int g = base + r * SCAN_THREADS + t;
+            c[r] = g < n ? counts[g * cs] : 0;   // g * cs < SCAN_MAX * COUNT_STRIDE
         }
 #pragma unroll
-        for (int r = 0; r < SCAN_REG; ++r)
-            if (g0 + r < b) {
-                begin[g0 + r] = run;
-                run += c[r];
-            }
+        for (int r = 0; r < SCAN_ROWS; ++r) {
+            const int g = base + r * SCAN_THREADS + t;
+            if (g < n) note_long(long_tiles, long_min, g, c[r]);
+            incl[r] = c[r];
+        }
+#pragma unroll
+        for (int r = 0; r < SCAN_ROWS; ++r) incl[r] = wave_incl_scan(incl[r]);
+        if (base > 0) __syncthreads();   // the previous rows' totals are read
+        if (lane == 63) {
+#pragma unroll
+            for (int r = 0; r < SCAN_ROWS; ++r) s_wave[r][wave] = incl[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < SCAN_ROWS; ++r) {
+            // lanes 0..15 of every wave scan the 16 wave totals (one DPP row)
+            int ws = s_wave[r][lane & (NW - 1)];
+            ws += __builtin_amdgcn_update_dpp(0, ws, 0x111, 0xf, 0xf, false);   // row_shr:1
+            ws += __builtin_amdgcn_update_dpp(0, ws, 0x112, 0xf, 0xf, false);   // row_shr:2
+            ws += __builtin_amdgcn_update_dpp(0, ws, 0x114, 0xf, 0xf, false);   // row_shr:4
+            ws += __builtin_amdgcn_update_dpp(0, ws, 0x118, 0xf, 0xf, false);   // row_shr:8
+            const int before = wave > 0 ? __builtin_amdgcn_readlane(ws, wave - 1) : 0;
+            const int total = __builtin_amdgcn_readlane(ws, NW - 1);
+            const int g = base + r * SCAN_THREADS + t;
+            if (g < n) begin[g] = carry + before + incl[r] - c[r];
+            carry += total;
+        }
     }
 }
 
