@@ -168,6 +168,15 @@ __device__ inline int wave_incl_scan(int v) {
     return v;
 }
 
+// Value of another lane by a DPP lane move (CTRL: a gfx9 DPP control, e.g.
+// quad_perm 0xB1 = lane xor 1, 0x4E = lane xor 2 inside each quad).  Every
+// lane of the wave must be active.
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+    return __builtin_bit_cast(
+            float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+
 // Summed-area-table index for a (side x side) corner grid stored either
 // row-major (period == 1) or phase-major with the given period: corners whose
 // row/column indices share residues mod `period` are contiguous, so one

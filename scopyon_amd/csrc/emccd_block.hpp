@@ -120,7 +120,7 @@ __device__ inline int queue_push(bool pred, int* counter) {
     const int lane = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
     int base = 0;
     if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
+    base = __builtin_amdgcn_readlane(base, leader);   // leader is uniform
     const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
     return pred ? base + rank : -1;

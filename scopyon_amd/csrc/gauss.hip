@@ -848,13 +848,15 @@ __global__ void __launch_bounds__(BT_THREADS) gauss_backward_tile_kernel(BwdTile
             float px = (vx[0] + vx[1]) + (vx[2] + vx[3]);
             float py = (vy[0] + vy[1]) + (vy[2] + vy[3]);
             float ps = (vs[0] + vs[1]) + (vs[2] + vs[3]);
-#pragma unroll
-            for (int m = 1; m < 4; m <<= 1) {
-                pI += __shfl_xor(pI, m);
-                px += __shfl_xor(px, m);
-                py += __shfl_xor(py, m);
-                ps += __shfl_xor(ps, m);
-            }
+            // sums over the quad (lanes xor 1, then xor 2) on DPP lane moves
+            pI += smx::dpp_f32<0xB1>(pI);
+            px += smx::dpp_f32<0xB1>(px);
+            py += smx::dpp_f32<0xB1>(py);
+            ps += smx::dpp_f32<0xB1>(ps);
+            pI += smx::dpp_f32<0x4E>(pI);
+            px += smx::dpp_f32<0x4E>(px);
+            py += smx::dpp_f32<0x4E>(py);
+            ps += smx::dpp_f32<0x4E>(ps);
             const float gs = emitter_outputs(pI, px, py, ps, q == 0 ? ee : -1, valid, v.z, v.w,
                                              A.P.inv_depth, A.gx, A.gy, A.gz, A.gI);
             if (valid && q == 0) s_gs[o] = (double)gs;
