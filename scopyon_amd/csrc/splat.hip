@@ -799,14 +799,17 @@ template <int QPW> constexpr int psort_cap() { return QPW == 1 ? 1024 : 2048; }
 // One 64-pixel part of an emitter's footprint rectangle inside a quadrant,
 // decoded for the lanes: pixel idx = p64 + lane of the rectangle (row-major,
 // width W) is valid for lane < lim; row = (idx * magic) >> 16 (exact for
-// idx < 256, W <= 16); its table element is kq + row * su + col * sv (bytes)
-// and its accumulator slot pix + row * AS + col (bytes * 8).
+// idx < 1024, W <= 32) and col = idx - row W, so its table element
+// kq + row su + col sv is kq + idx sv + row (su - W sv) (bytes) and its
+// accumulator slot pix + 8 (row AS + col) is pix + 8 idx + row 8 (AS - W):
+// two 24-bit multiply-adds each, col never formed.
 struct __align__(16) PartRec {
     int kq;          // table byte offset (32-bit form) or low half of the byte address
     int magic_lim;   // ceil(65536 / W) | lim << 24 (valid lanes, 0..64)
-    int negw;        // -W
+    int dk;          // su - W sv: table bytes per rectangle row beyond idx sv
     int p64;         // first rectangle pixel of the part
-    int su, sv;      // table bytes per pixel along u, v
+    int da;          // 8 (AS - W): accumulator bytes per rectangle row beyond 8 idx
+    int sv;          // table bytes per pixel along v
     int pix;         // accumulator byte offset of the rectangle's first pixel
     int kq_hi;       // high half of the byte address (64-bit form)
 };
@@ -1031,9 +1034,9 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                         PartRec& R = pr[pos];
                         R.kq = O32 ? (int)(unsigned)kb : (int)(unsigned)ka;
                         R.magic_lim = magic | (lim << 24);
-                        R.negw = -W;
+                        R.dk = (su - W * sv) * (int)sizeof(KT);
                         R.p64 = p * 64;
-                        R.su = su * (int)sizeof(KT);
+                        R.da = (AS - W) * 8;
                         R.sv = sv * (int)sizeof(KT);
                         R.pix = pix;
                         R.kq_hi = O32 ? 0 : (int)(unsigned)(ka >> 32);
@@ -1046,9 +1049,9 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                         PartRec& R = pr[lane];
                         R.kq = O32 ? 0 : (int)(unsigned)(unsigned long long)kbase;
                         R.magic_lim = 0;
-                        R.negw = 0;
+                        R.dk = 0;
                         R.p64 = 0;
-                        R.su = 0;
+                        R.da = 0;
                         R.sv = 0;
                         R.pix = 0;
                         R.kq_hi = O32 ? 0 : (int)(unsigned)((unsigned long long)kbase >> 32);
@@ -1066,27 +1069,26 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
 #pragma unroll
                         for (int s = 0; s < NS; ++s) {
                             const PartRec& R = pr[b + s];
-                            const int4 h = *(const int4*)&R.kq;
-                            const int4 g = *(const int4*)&R.su;
+                            const int4 h = *(const int4*)&R.kq;   // kq, magic_lim, dk, p64
+                            const int4 g = *(const int4*)&R.da;   // da, sv, pix, kq_hi
                             if (TAPS == 1) wv[s] = pw[b + s];
                             const unsigned idx = (unsigned)lane + (unsigned)h.w;
                             const bool ok = lane24 < (unsigned)h.y;
                             const int row = (int)(__umul24(idx, (unsigned)h.y) >> 16);
-                            const int col = __mul24(row, h.z) + (int)idx;
                             const char* kp;
                             if (O32) {
-                                int off = __mul24(col, g.y) + (__mul24(row, g.x) + h.x);
+                                int off = __mul24(row, h.z) + (__mul24((int)idx, g.y) + h.x);
                                 off = ok ? off : 0;
                                 kp = kbase + (unsigned)off;
                             } else {
-                                int off = __mul24(col, g.y) + __mul24(row, g.x);
+                                int off = __mul24(row, h.z) + __mul24((int)idx, g.y);
                                 off = ok ? off : 0;
                                 kp = (const char*)(((unsigned long long)(unsigned)g.w << 32) |
                                                    (unsigned)h.x) + off;
                             }
                             if (TAPS == 4) kc[s] = smx::gload4((const float*)kp);
                             else kv[s] = smx::gload((const KT*)kp);
-                            const unsigned l = (unsigned)__mul24(row, AS * 8) + (unsigned)g.z + ((unsigned)col << 3);
+                            const unsigned l = (unsigned)__mul24(row, g.x) + ((unsigned)g.z + (idx << 3));
                             la[s] = ok ? l : spill;
                         }
 #pragma unroll
