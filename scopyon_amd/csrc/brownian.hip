@@ -70,6 +70,26 @@ __global__ void brownian_predrawn_kernel(int n, int ndim, int nsteps, int nstate
     }
 }
 
+// Box-Muller normals of the Philox paths in float32 from 53-bit uniforms:
+// u1 in (0, 1] converted to float still reaches 2^-53, so the tails reach
+// |z| ~ 8.6 as in float64, and the ~1e-7 relative rounding of a displacement
+// is far below anything physical; float64 log / sqrt / sincos took most of a
+// step's time (640,000 emitters: ~10 us).
+__device__ inline void normal_pair(const smx::u32x4& r, double& z0, double& z1) {
+    const float u1 = (float)smx::u01_53_open0(r.x, r.y);
+    const float u2 = (float)smx::u01_53(r.z, r.w);
+    const float rad = sqrtf(-2.0f * logf(u1));
+    float s, c;
+    sincospif(2.0f * u2, &s, &c);
+    z0 = (double)(rad * c);
+    z1 = (double)(rad * s);
+}
+__device__ inline double normal_one(const smx::u32x4& r) {
+    const float u1 = (float)smx::u01_53_open0(r.x, r.y);
+    const float u2 = (float)smx::u01_53(r.z, r.w);
+    return (double)(sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2));
+}
+
 // One step for structure-of-arrays coordinates (ndim <= 3).
 __global__ void brownian_philox_kernel(int n, int ndim, double* __restrict__ x0,
                                        double* __restrict__ x1, double* __restrict__ x2,
@@ -87,18 +107,12 @@ __global__ void brownian_philox_kernel(int n, int ndim, double* __restrict__ x0,
     double sc = sqrt(2.0 * Di * dt);
     smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u},
                                       (uint32_t)seed, (uint32_t)(seed >> 32));
-    double u1 = smx::u01_53_open0(r.x, r.y);
-    double u2 = smx::u01_53(r.z, r.w);
-    double rad = sqrt(-2.0 * log(u1));
-    double s, c;
-    sincos(6.283185307179586 * u2, &s, &c);
-    double z[3] = {rad * c, rad * s, 0.0};
+    double z[3] = {0.0, 0.0, 0.0};
+    normal_pair(r, z[0], z[1]);
     if (ndim > 2) {
         smx::u32x4 q = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u},
                                           (uint32_t)seed, (uint32_t)(seed >> 32));
-        double v1 = smx::u01_53_open0(q.x, q.y);
-        double v2 = smx::u01_53(q.z, q.w);
-        z[2] = sqrt(-2.0 * log(v1)) * cos(6.283185307179586 * v2);
+        z[2] = normal_one(q);
     }
     double* xs[3] = {x0, x1, x2};
     double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
@@ -139,17 +153,11 @@ __global__ void brownian_states_kernel(int n, int ndim, double* __restrict__ x0,
     s = s < 0 ? 0 : (s >= nstates ? nstates - 1 : s);
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u}, k0, k1);
-    double u1 = smx::u01_53_open0(r.x, r.y);
-    double u2 = smx::u01_53(r.z, r.w);
-    double rad = sqrt(-2.0 * log(u1));
-    double sn, cs;
-    sincos(6.283185307179586 * u2, &sn, &cs);
-    double z[3] = {rad * cs, rad * sn, 0.0};
+    double z[3] = {0.0, 0.0, 0.0};
+    normal_pair(r, z[0], z[1]);
     if (ndim > 2) {
         smx::u32x4 q = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u}, k0, k1);
-        double v1 = smx::u01_53_open0(q.x, q.y);
-        double v2 = smx::u01_53(q.z, q.w);
-        z[2] = sqrt(-2.0 * log(v1)) * cos(6.283185307179586 * v2);
+        z[2] = normal_one(q);
     }
     double* xs[3] = {x0, x1, x2};
     const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
