@@ -176,6 +176,40 @@ def test_long_tile_lists_match_oracle(gpu):
                 assert numpy.array_equal(again.view(numpy.uint64), got.view(numpy.uint64)), ts
 
 
+def test_mid_length_tile_lists_match_oracle(gpu):
+    """700 and 1,500 emitters on one 64 x 64 frame: a 32 px tile's list then
+    holds ~360 or ~780 entries and a 64 px tile's all of them, beyond one
+    entry per thread of the block, so the tile pass sorts them with the
+    merge-rank sort (wave-sorted runs ranked against each other) -- frames
+    equal the oracle to rounding and are bit-identical across tile sides and
+    from run to run."""
+    from scopyon_amd.batched import BatchedSimulator
+    shape = (64, 64)
+    p = oracle_params({'detector': {'image_size': list(shape)}, 'magnification': 100.0})
+    for k in (700, 1500):
+        print('mid-length lists: {} emitters'.format(k), flush=True)
+        ref = first = None
+        for ts in (16, 32, 64):
+            sim = BatchedSimulator(_cfg(shape), [shape], [k], dt=0.033, seed=23, tile_side=ts,
+                                   table_precision='float64')
+            sim.form()
+            assert sim.status() == 0
+            if ref is None:
+                u, v = sim.u.cpu().numpy(), sim.v.cpu().numpy()
+                parts = numpy.column_stack([numpy.zeros(k), u, v, numpy.arange(k), numpy.ones(k)])
+                ref, _, _ = O.molecule_plane(p, O.PSFCache(p), parts, shape, numpy.zeros(3), 0.033,
+                                             {}, None, None)
+            got = sim.frame(0, 'photons').cpu().numpy()
+            numpy.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12 * ref.max(),
+                                          err_msg=str(ts))
+            if first is None:
+                first = got
+            assert numpy.array_equal(got.view(numpy.uint64), first.view(numpy.uint64)), ts
+            sim.form()
+            again = sim.frame(0, 'photons').cpu().numpy()
+            assert numpy.array_equal(again.view(numpy.uint64), got.view(numpy.uint64)), ts
+
+
 @pytest.mark.parametrize('precision', ['float64', 'float32', 'folded'])
 def test_table_address_forms_agree(gpu, monkeypatch, precision):
     """The tile pass addresses the phase table with 32-bit byte offsets from
