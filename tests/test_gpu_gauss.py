@@ -404,3 +404,24 @@ def test_mfma_forward_deterministic_and_unsorted_lists(gpu):
             assert m.check_status() & 1
         ref = G.render(shapes, xs, ys, numpy.zeros(len(xs)), I, numpy.zeros(len(xs), int), **kw)[0]
         numpy.testing.assert_allclose(a.view(64, 64).cpu().numpy(), ref, rtol=2e-5)
+
+
+def test_mfma_forward_merge_sorted_lists(gpu):
+    """2,000 emitters on 64^2 (four 32 px tiles): ~700 entries per tile list,
+    sorted in the tile pass by the merge-rank sort's four 256-entry runs --
+    repeatable bits and the image of the float64 renderer."""
+    from scopyon_amd.gauss import GaussianModel
+    shapes = [(64, 64)]
+    rng = numpy.random.RandomState(29)
+    kw = dict(sigma=1.2, inv_depth=0.0, background=1.0, radius=6)
+    n = 2000
+    x, y = rng.uniform(0, 64, n), rng.uniform(0, 64, n)
+    I = rng.uniform(1.0, 2.0, n)
+    m = GaussianModel(shapes, method='mfma-f32', **kw)
+    args = (_dev(x), _dev(y), None, _dev(I))
+    a = m.forward(*args).clone()
+    b = m.forward(*args).clone()
+    assert torch.equal(a, b)
+    assert m.check_status() == 0
+    ref = G.render(shapes, x, y, numpy.zeros(n), I, numpy.zeros(n, int), **kw)[0]
+    numpy.testing.assert_allclose(a.view(64, 64).cpu().numpy(), ref, rtol=2e-5)
