@@ -174,6 +174,13 @@ int smx_splat_accumulate(int n, const double* u, const double* v, const double* 
  * workload and keeps the fastest.  No reference counterpart (the reference
  * loops over emitters on the whole frame, _epifm.py:215-282). */
 #define SMX_TILE_Q32 0x100
+/* Or-ed into the tile side: this workspace was binned before by a call with
+ * the same FOV layout, tile side, emitter count and list capacity, so its
+ * tile counters are back at zero (the fill counts them down) and its tile ->
+ * FOV map is current; the binning then skips their initialisation (one
+ * launch).  Only with fewer than 65,536 tiles (the one-workgroup counter
+ * scan); ignored otherwise.  No reference counterpart. */
+#define SMX_TILE_WARM 0x200
 int smx_splat_expected_ts(int n, const double* u, const double* v, const double* w,
                           const int* key, const int* fov_id, const smx_fov_t* fovs_dev, int nfov,
                           int n_tiles, const double* sat, size_t sat_stride, int n_slots,

@@ -217,8 +217,10 @@ class BatchedSimulator:
 
     def _tile_arg(self):
         """The C ABI's tile argument: the side, with SMX_TILE_Q32 for 32 x 32
-        quadrants."""
-        return self.tile_side | (0x100 if self.quadrant == 32 else 0)
+        quadrants and SMX_TILE_WARM once this workspace has been binned with
+        the current layout (its counters are back at zero, its tile map is
+        current: the binning skips their initialisation)."""
+        return self.tile_side | (0x100 if self.quadrant == 32 else 0) | (0x200 if self._warm else 0)
 
     def _set_geometry(self, ts):
         """FOV descriptors counted in tiles of side ts."""
@@ -236,8 +238,10 @@ class BatchedSimulator:
         self.fovs_dev = torch.from_numpy(numpy.frombuffer(raw, dtype=numpy.uint8).copy()).to(
             self.device)
         self.tile_side = ts
+        self._warm = False
 
     def _set_workspace(self):
+        self._warm = False
         self.ws = torch.empty(
             _lib.lib().smx_splat_workspace_bytes(self.n, self.n_tiles, self.capacity),
             dtype=torch.uint8, device=self.device)
@@ -314,6 +318,7 @@ class BatchedSimulator:
                       self.capacity, 0, self.photons.data_ptr(), self.ws.data_ptr(),
                       self.ws.numel(), self.splatter.status.data_ptr(), _device.stream(),
                       stages[entry], self._tile_arg())
+            self._warm |= bool(stages[entry] & 1)
             return
         _lib.call(entry + '_ts', self.n, self.u.data_ptr(), self.v.data_ptr(),
                   self.w.data_ptr(), self.key.data_ptr(), self.fov_id.data_ptr(),
@@ -322,6 +327,7 @@ class BatchedSimulator:
                   _device.ptr(self.Ktab), self.k_format, t.m, self.photons.data_ptr(),
                   self.ws.data_ptr(), self.ws.numel(), self.splatter.status.data_ptr(),
                   _device.stream(), self._tile_arg())
+        self._warm |= entry != 'smx_splat_accumulate'
 
     def detect(self):
         """EMCCD + ADC of every frame, then advance the frame counter."""
@@ -351,6 +357,7 @@ class BatchedSimulator:
                       _device.ptr(self.gain_map), _device.ptr(self.offset_map), self.det,
                       self.seed, 0, 0, self.counter.data_ptr(), _device.ptr(self.E),
                       self.adc.data_ptr(), f32, stages[entry], self._tile_arg())
+            self._warm |= bool(stages[entry] & 1)
             return
         _lib.call(entry, self.n, self.u.data_ptr(), self.v.data_ptr(), self.w.data_ptr(),
                   self.key.data_ptr(), self.fov_id.data_ptr(), self.fovs_dev.data_ptr(),
@@ -361,6 +368,7 @@ class BatchedSimulator:
                   _device.ptr(self.offset_map), self.det, self.seed, 0, 0,
                   self.counter.data_ptr(), _device.ptr(self.E), self.adc.data_ptr(), f32,
                   self._tile_arg())
+        self._warm |= entry == 'smx_splat_detect_ts'
 
     def stages(self):
         """The launches of one step, in order, as callables (for timing)."""

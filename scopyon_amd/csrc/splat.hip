@@ -1322,7 +1322,8 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     }
     const smx::PhaseFold PF = smx::make_phase_fold(ntab, sat_period > 0 ? sat_period : 1);
     const bool q32 = (tile_side & SMX_TILE_Q32) != 0;
-    const int side = tile_side & ~SMX_TILE_Q32;
+    const bool warm = (tile_side & SMX_TILE_WARM) != 0;
+    const int side = tile_side & ~(SMX_TILE_Q32 | SMX_TILE_WARM);
     const int lts = side == 16 ? 4 : side == 32 ? 5 : side == 64 ? 6 : -1;
     if (lts < 0 || (!K && (lts != 5 || q32)) || (q32 && lts == 4)) {
         smx::set_error("smx_splat_expected: tile side %d (16, 32 or 64 with phase tables, "
@@ -1357,10 +1358,14 @@ static int splat_run(int n, const double* u, const double* v, const double* w, c
     const int cs = K ? smx::agg_count_stride(2LL * n, n_tiles) : smx::count_stride(2LL * n, n_tiles);
 
     if (stages & 1) {
-        hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256), 0,
-                           stream, nfov, fovs_dev, n_tiles, ws.tile_count, cs, ws.tile_fov,
-                           ws.long_tiles);
-        SMX_CHECK_LAUNCH("tiles_init_kernel");
+        // a warm workspace's counters are at zero and its tile map current;
+        // the one-workgroup scan clears the long-list count itself
+        if (!(warm && n_tiles + 1 <= smx::SCAN_MAX)) {
+            hipLaunchKernelGGL(smx::tiles_init_kernel, dim3((n_tiles + 1 + 255) / 256), dim3(256),
+                               0, stream, nfov, fovs_dev, n_tiles, ws.tile_count, cs, ws.tile_fov,
+                               ws.long_tiles);
+            SMX_CHECK_LAUNCH("tiles_init_kernel");
+        }
         if (n > 0) {
             if (K) {
                 // records first; they count their tiles as they are written

@@ -93,6 +93,10 @@ __global__ void __launch_bounds__(SCAN_THREADS) scan_counts_kernel(const int* __
     static_assert(NW == 16, "one DPP row of 16 lanes scans the wave totals");
     __shared__ int s_wave[SCAN_ROWS][NW];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // the long-list count starts at zero here (a binning that skips the
+    // counter initialisation relies on it)
+    if (t == 0 && long_tiles) long_tiles[0] = 0;
+    __syncthreads();
     int carry = 0;   // sum of the rows before
     for (int base = 0; base < n; base += SCAN_THREADS * SCAN_ROWS) {
         int c[SCAN_ROWS], incl[SCAN_ROWS];
