@@ -169,7 +169,8 @@ class BatchedSimulator:
         # 'float64': the phase tables themselves (same values as the SAT path);
         # 'float32': their float32 copy (half the table traffic); 'folded': the
         # symmetric float32 fold (1/8 of the table, L2-resident).  With float32
-        # each pixel contribution is rounded to float32 (relative error <= 2^-24).
+        # each pixel contribution is a float32 product of the table value and
+        # the emitter weight (relative error <= ~3 x 2^-24), summed in float64.
         if table_precision not in ('float64', 'float32', 'folded'):
             raise ValueError('table_precision must be float64, float32 or folded')
         self.k_format = 0

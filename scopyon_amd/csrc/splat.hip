@@ -785,8 +785,8 @@ __device__ __forceinline__ double sat_pixel(const FastArgs& A, const smx_fov_t& 
 // list order, with no branch: lanes outside a part add into their own spill
 // slot.  Every pixel sums its emitters in emitter order, and the lanes of one
 // part touch distinct pixels.  KT is the phase-table element type: double
-// (values identical to the SAT path) or float (each contribution rounded to
-// float32, relative error <= 2^-24).
+// (values identical to the SAT path) or float (table value times the weight
+// in float32: each contribution within ~3 x 2^-24 relative).
 // Quadrant side QS (one wave each): 16, or 32.  Accumulator row stride QS + 1
 // (doubles).  ds_add_f64 serves 32 lanes per 256 bytes of banks; with 16 the
 // rows of a narrow part share banks, 17 spreads them (measured: 17 is the
@@ -1008,6 +1008,10 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
             } else {
                 PartRec* pr = s_pr[wave];
                 double* pw = s_pw[wave];
+                // float tables: the weight in float32 too, the product converted
+                // to float64 for the add (contribution within ~3 2^-24 relative)
+                constexpr bool F32W = std::is_same<KT, float>::value && TAPS == 1;
+                float* pwf = reinterpret_cast<float*>(pw);
                 float4* pc = s_pc[wave];
                 const int magic = (65536 + W - 1) / W;
                 const double w = lane < nc ? s_w[lane] : 0.0;
@@ -1041,6 +1045,7 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                         R.pix = pix;
                         R.kq_hi = O32 ? 0 : (int)(unsigned)(ka >> 32);
                         if (TAPS == 4) pc[pos] = cw;
+                        else if (F32W) pwf[pos] = (float)w;
                         else pw[pos] = w;
                     }
                     const int cnt = min(PMAX, total - rb);
@@ -1056,6 +1061,7 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                         R.pix = 0;
                         R.kq_hi = O32 ? 0 : (int)(unsigned)((unsigned long long)kbase >> 32);
                         if (TAPS == 4) pc[lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        else if (F32W) pwf[lane] = 0.0f;
                         else pw[lane] = 0.0;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1065,13 +1071,15 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                         KT kv[NS];
                         float4 kc[NS];   // LUT cell: the four taps of a pixel
                         double wv[NS];
+                        float wf[NS];
                         unsigned la[NS];
 #pragma unroll
                         for (int s = 0; s < NS; ++s) {
                             const PartRec& R = pr[b + s];
                             const int4 h = *(const int4*)&R.kq;   // kq, magic_lim, dk, p64
                             const int4 g = *(const int4*)&R.da;   // da, sv, pix, kq_hi
-                            if (TAPS == 1) wv[s] = pw[b + s];
+                            if (F32W) wf[s] = pwf[b + s];
+                            else if (TAPS == 1) wv[s] = pw[b + s];
                             const unsigned idx = (unsigned)lane + (unsigned)h.w;
                             const bool ok = lane24 < (unsigned)h.y;
                             const int row = (int)(__umul24(idx, (unsigned)h.y) >> 16);
@@ -1099,7 +1107,8 @@ splat_phase_kernel(FastArgs A, DetectArgs D) {
                                                   c.w * kc[s].w;
                                 lds_add((double*)((char*)s_acc + la[s]), (double)val);
                             } else {
-                                lds_add((double*)((char*)s_acc + la[s]), (double)kv[s] * wv[s]);
+                                lds_add((double*)((char*)s_acc + la[s]),
+                                        F32W ? (double)((float)kv[s] * wf[s]) : (double)kv[s] * wv[s]);
                             }
                         }
                     }
