@@ -226,6 +226,23 @@ __device__ inline void lds_bitonic_sort(int* a, int n_pow2) {
 // shuffle), inside the lane for j >= 64 -- and the lower of the pair keeps
 // the minimum when p & k == 0, the maximum otherwise.  The sorted list is
 // written to out[0 .. 64 * NPL).
+// Value of lane (lane xor j) for the sorts below: DPP lane moves for j < 16
+// (quad_perm [1,0,3,2] and [2,3,0,1]; for 4 and 8 a quad or row reversal
+// followed by the half-row mirror, since 7 - i = i ^ 7 inside 8 lanes and
+// 15 - i = i ^ 15 inside 16), a shuffle (ds_bpermute) for 16 and 32.  j is a
+// compile-time constant after unrolling; every lane of the wave is active.
+__device__ inline int lane_xor(int v, int j) {
+    if (j == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);
+    if (j == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);
+    if (j == 4)
+        return __builtin_amdgcn_update_dpp(
+                0, __builtin_amdgcn_update_dpp(0, v, 0x1B, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
+    if (j == 8)
+        return __builtin_amdgcn_update_dpp(
+                0, __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
+    return __shfl_xor(v, j);
+}
+
 template <int NPL>
 __device__ inline void wave_sort(const int* src, int count, int lane, int* out) {
     int v[NPL];
@@ -252,7 +269,9 @@ __device__ inline void wave_sort(const int* src, int count, int lane, int* out) 
                 const bool lower = (lane & j) == 0;
 #pragma unroll
                 for (int r = 0; r < NPL; ++r) {
-                    const int b = __shfl_xor(v[r], j);
+                    // DPP moves when several registers interleave; one element
+                    // per lane waits on their hazards (measured slower)
+                    const int b = NPL >= 2 ? lane_xor(v[r], j) : __shfl_xor(v[r], j);
                     const bool up = ((r * 64 + lane) & k) == 0;
                     v[r] = (lower == up) ? min(v[r], b) : max(v[r], b);
                 }
