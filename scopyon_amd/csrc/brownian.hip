@@ -105,12 +105,12 @@ __global__ void brownian_philox_kernel(int n, int ndim, double* __restrict__ x0,
     if (step_dev) step += *step_dev;
     double Di = D ? D[i] : D_scalar;
     double sc = sqrt(2.0 * Di * dt);
-    smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u},
+    smx::u32x4 r = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u},
                                       (uint32_t)seed, (uint32_t)(seed >> 32));
     double z[3] = {0.0, 0.0, 0.0};
     normal_pair(r, z[0], z[1]);
     if (ndim > 2) {
-        smx::u32x4 q = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u},
+        smx::u32x4 q = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u},
                                           (uint32_t)seed, (uint32_t)(seed >> 32));
         z[2] = normal_one(q);
     }
@@ -152,11 +152,11 @@ __global__ void brownian_states_kernel(int n, int ndim, double* __restrict__ x0,
     int s = state ? state[i] : 0;
     s = s < 0 ? 0 : (s >= nstates ? nstates - 1 : s);
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-    smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u}, k0, k1);
+    smx::u32x4 r = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 0u}, k0, k1);
     double z[3] = {0.0, 0.0, 0.0};
     normal_pair(r, z[0], z[1]);
     if (ndim > 2) {
-        smx::u32x4 q = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u}, k0, k1);
+        smx::u32x4 q = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 1u}, k0, k1);
         z[2] = normal_one(q);
     }
     double* xs[3] = {x0, x1, x2};
@@ -171,7 +171,7 @@ __global__ void brownian_states_kernel(int n, int ndim, double* __restrict__ x0,
         xs[d][i] = v;
     }
     if (pacc && state) {
-        smx::u32x4 t = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 2u}, k0, k1);
+        smx::u32x4 t = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, step, 0xB0A1u, 2u}, k0, k1);
         const double u = smx::u01_53(t.x, t.y);
         const double* row = pacc + (size_t)s * nstates;
         int k = 0;
@@ -186,9 +186,9 @@ __global__ void uniform_init_kernel(int n, int ndim, double* __restrict__ x0,
                                     double hi2, uint64_t seed, uint32_t seq, uint32_t id_base) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    smx::u32x4 r = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, seq, 0x1217u, 0u},
+    smx::u32x4 r = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, seq, 0x1217u, 0u},
                                       (uint32_t)seed, (uint32_t)(seed >> 32));
-    smx::u32x4 q = smx::philox4x32_10(smx::u32x4{id_base + (uint32_t)i, seq, 0x1217u, 1u},
+    smx::u32x4 q = smx::philox4x32_10_dev(smx::u32x4{id_base + (uint32_t)i, seq, 0x1217u, 1u},
                                       (uint32_t)seed, (uint32_t)(seed >> 32));
     double u[3] = {smx::u01_53(r.x, r.y), smx::u01_53(r.z, r.w), smx::u01_53(q.x, q.y)};
     double* xs[3] = {x0, x1, x2};

@@ -55,6 +55,27 @@ SMX_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
     return c;
 }
 
+// The same rounds on the device with "hi ^ word ^ key" as one v_bitop3_b32
+// (three-input bitwise op, truth table 0x96 = XOR of the three; gfx950 has
+// no v_xor3_b32, and the compiler otherwise emits two XORs per word).
+// Results are identical to philox4x32_10.
+__device__ inline u32x4 philox4x32_10_dev(u32x4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)M0 * c.x;
+        uint64_t p1 = (uint64_t)M1 * c.z;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c = u32x4{(uint32_t)__builtin_amdgcn_bitop3_b32(hi1, c.y, k0, 0x96), lo1,
+                  (uint32_t)__builtin_amdgcn_bitop3_b32(hi0, c.w, k1, 0x96), lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
 // 53-bit uniform in [0, 1) from two 32-bit words.
 SMX_HD double u01_53(uint32_t a, uint32_t b) {
     uint64_t v = ((uint64_t)a << 21) ^ (uint64_t)(b >> 11);

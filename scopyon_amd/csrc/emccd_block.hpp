@@ -29,7 +29,7 @@ __device__ inline float u01f_open0(uint32_t w) { return 1.0f - u01f(w); }       
 struct PixelRng {
     uint32_t id, frame, k0, k1;
     __device__ smx::u32x4 block(uint32_t b) const {
-        return smx::philox4x32_10(smx::u32x4{id, frame, 0x0E3CCD01u, b}, k0, k1);
+        return smx::philox4x32_10_dev(smx::u32x4{id, frame, 0x0E3CCD01u, b}, k0, k1);
     }
 };
 
